@@ -1,0 +1,141 @@
+/*
+ * lumen_sgbm.h - C ABI of the MI355X SGBM matcher (liblumen_sgbm.so).
+ *
+ * This is the drop-in boundary for the reference's matcher. Everything is extern "C", takes
+ * plain pointers and sizes, never throws, and returns an int status:
+ *   LUMEN_OK (0), LUMEN_EINVAL (1), LUMEN_EDEVICE (2), LUMEN_ENOMEM (3).
+ * The text of the last error of a handle is returned by lumen_sgbm_last_error(); errors that
+ * occur before a handle exists are reported by lumen_last_error().
+ *
+ * Citations are into the reference (Pfactorial01/LumenStereo); each entry point names the
+ * interface it replaces.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LUMEN_OK 0
+#define LUMEN_EINVAL 1
+#define LUMEN_EDEVICE 2
+#define LUMEN_ENOMEM 3
+
+/* Largest disparity range (maxDisparity - minDisparity); replaces kSgbmMaxDisparityRange
+ * (include/stereo/sgbm_limits.h:9). */
+#define LUMEN_MAX_DISPARITY_RANGE 512
+
+/* Mirrors stereo::StereoParams field by field (include/stereo/stereo_params.h:57-82) as int32.
+ * useGPU is accepted and ignored, as in the reference (nothing reads it). useSharedMemory is
+ * accepted and does not change results (src/sgbm_gpu.cu:235-285 gives identical costs either
+ * way). matchingCostMode: 0 = SAD, 1 = Census (include/stereo/stereo_params.h:20-23). */
+typedef struct lumen_stereo_params {
+    int32_t minDisparity;
+    int32_t maxDisparity;
+    int32_t blockSize;
+    int32_t P1;
+    int32_t P2;
+    int32_t preFilterCap;
+    int32_t uniquenessRatio;
+    int32_t disp12MaxDiff;
+    int32_t speckleWindowSize;
+    int32_t speckleRange;
+    int32_t useGPU;
+    int32_t useSharedMemory;
+    int32_t numDirections;
+    int32_t matchingCostMode;
+} lumen_stereo_params;
+
+typedef struct lumen_sgbm lumen_sgbm;
+
+/* Writes the StereoParams defaults (include/stereo/stereo_params.h:59-82). */
+int lumen_sgbm_default_params(lumen_stereo_params* out);
+
+/* Replaces StereoSGBM::StereoSGBM(const StereoParams&) (src/sgbm_gpu.cu:100-107). No device
+ * call is made until the first compute; the stream and events are created then. */
+int lumen_sgbm_create(const lumen_stereo_params* params, lumen_sgbm** out);
+
+/* Replaces StereoSGBM::~StereoSGBM (src/sgbm_gpu.cu:109-113). NULL is accepted. */
+int lumen_sgbm_destroy(lumen_sgbm* h);
+
+/* Replace StereoSGBM::setParams / getParams (src/sgbm_gpu.cu:115-117,
+ * include/stereo/sgbm_gpu.h:47). */
+int lumen_sgbm_set_params(lumen_sgbm* h, const lumen_stereo_params* params);
+int lumen_sgbm_get_params(const lumen_sgbm* h, lumen_stereo_params* out);
+
+/* Replaces StereoSGBM::compute(const cv::Mat&, const cv::Mat&, cv::Mat&)
+ * (src/sgbm_gpu.cu:147-195). Host images of width x height with `channels` 1 (grayscale) or 3
+ * (BGR, converted to gray on the host as the reference's cvtColor does); any other channel
+ * count is treated as grayscale bytes. Row strides are in bytes (0 = packed). Writes int16
+ * disparity in 1/16 px (-1 = invalid) to `out` (stride in bytes, 0 = packed). Blocking.
+ * Validation order and messages follow src/sgbm_gpu.cu:148-166. */
+int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
+                       const uint8_t* right, size_t right_stride, int width, int height,
+                       int channels, int16_t* out, size_t out_stride);
+
+/* Device-resident form of lumen_sgbm_compute (no reference counterpart: the reference always
+ * copies from host memory). d_left/d_right are packed grayscale images already in device
+ * memory, d_out a packed int16 device buffer. Work is enqueued on the handle's stream; when
+ * `sync` is non-zero the call waits for it. */
+int lumen_sgbm_compute_device(lumen_sgbm* h, const uint8_t* d_left, const uint8_t* d_right,
+                              int width, int height, int16_t* d_out, int sync);
+
+/* The hipStream_t the handle launches on (as void*), creating it if needed. */
+int lumen_sgbm_get_stream(lumen_sgbm* h, void** stream_out);
+
+/* Stage times of the last completed compute in milliseconds, from hipEvents on the handle's
+ * stream: out[0] total (as getLastComputeTimeMs, src/sgbm_gpu.cu:177-191: upload through
+ * post-processing), out[1] cost (getCostComputeTimeMs), out[2] aggregation
+ * (getAggregationTimeMs), out[3] post-processing (getPostProcessTimeMs: speckle and median),
+ * out[4] path sum + WTA + LR check, out[5] upload. Waits for the last compute. */
+int lumen_sgbm_get_timings(lumen_sgbm* h, float out[6]);
+
+/* Average of the same six stage times over the last `ncalls` completed computes (at most 64
+ * are kept). Waits for them. */
+int lumen_sgbm_get_timings_avg(lumen_sgbm* h, int ncalls, float out[6]);
+
+/* Text of the last error on this handle ("" if none). Valid until the next call on it. */
+const char* lumen_sgbm_last_error(const lumen_sgbm* h);
+
+/* Text of the last error raised without a handle (e.g. by lumen_sgbm_create), per thread. */
+const char* lumen_last_error(void);
+
+/* Parity hook (no reference counterpart): copies one stage of the last compute into host
+ * memory in the reference's layout (idx = (y*W + x)*D + d, D = maxDisparity - minDisparity).
+ * Stages: */
+#define LUMEN_STAGE_COST 0        /* uint8  W*H*D */
+#define LUMEN_STAGE_PATH 1        /* uint16 W*H*D, path index in `path` (0..npaths-1) */
+#define LUMEN_STAGE_SUM 2         /* uint16 W*H*D, sum of all paths mod 2^16 */
+#define LUMEN_STAGE_WTA_INT 3     /* int16  W*H, integer best disparity index of the left WTA */
+#define LUMEN_STAGE_DISP_LEFT 4   /* int16  W*H, left WTA in 1/16 px before the LR check */
+#define LUMEN_STAGE_DISP_RIGHT 5  /* int16  W*H, right WTA in 1/16 px */
+#define LUMEN_STAGE_AFTER_LR 6    /* int16  W*H, after the LR check */
+#define LUMEN_STAGE_AFTER_SPECKLE 7 /* int16 W*H, after the speckle filter */
+#define LUMEN_STAGE_OUTPUT 8      /* int16  W*H, final output (after the median) */
+/* Path order: 0 LR, 1 RL, 2 TB, 3 BT, 4 TLBR, 5 BRTL, 6 TRBL, 7 BLTR
+ * (the reference's launch order, src/sgbm_gpu.cu:321-421). */
+int lumen_sgbm_dump_stage(lumen_sgbm* h, int stage, int path, void* host, size_t bytes);
+
+/* Number of SGM paths the handle's parameters select (4, or 8 when numDirections >= 8;
+ * src/sgbm_gpu.cu:359). */
+int lumen_sgbm_num_paths(const lumen_sgbm* h);
+
+/* Device bytes the handle needs for a width x height pair with its current parameters:
+ * the padded cost volume, the per-path uint16 buffers and the image-sized buffers. */
+int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int height,
+                         size_t* bytes_out);
+
+/* Deterministic synthetic stereo pair (no reference counterpart; the benchmark scenes):
+ * band-limited noise left image, slanted planes plus fronto-parallel occluders as ground
+ * truth with disparities in [dmin, dmax], right image = left warped by -d with occlusions
+ * filled by fresh noise, gain/bias/noise applied. lowtex_frac of the area gets its texture
+ * std reduced to 5. gt may be NULL. Same seed gives the same bytes on every machine that runs
+ * this build. */
+int lumen_synth_pair(int width, int height, float dmin, float dmax, uint64_t seed,
+                     float lowtex_frac, uint8_t* left, uint8_t* right, float* gt);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,339 @@
+"""MI355X SGBM matcher: Python host mirror of the reference's stereo::StereoSGBM interface.
+
+The compute path is liblumen_sgbm.so (hand-written gfx950 HIP kernels behind a C ABI, declared
+in include/lumen_sgbm.h). This module only binds that ABI with ctypes and mirrors the
+reference's types so that Python callers and tests read like the reference's own:
+
+* ``StereoParams`` / ``MatchingCostMode`` mirror include/stereo/stereo_params.h:20-104.
+* ``StereoSGBM`` mirrors include/stereo/sgbm_gpu.h:29-91 (compute, setParams, getParams and the
+  four timing getters).
+* ``StereoException`` is raised for the validation errors of src/sgbm_gpu.cu:148-166, and
+  ``GpuException`` for runtime/device errors (the reference's CudaException,
+  include/stereo/common.h:31-48).
+
+There is no fallback: if the shared library is missing, importing the binding raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import enum
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "MatchingCostMode",
+    "StereoParams",
+    "StereoSGBM",
+    "StereoException",
+    "GpuException",
+    "kSgbmMaxDisparityRange",
+    "middleburyMatcherMaxDisparity",
+    "synth_pair",
+    "library",
+    "LIB_PATH",
+    "STAGE",
+]
+
+LIB_PATH = Path(__file__).resolve().parent / "liblumen_sgbm.so"
+
+# include/stereo/sgbm_limits.h:9
+kSgbmMaxDisparityRange = 512
+
+LUMEN_OK, LUMEN_EINVAL, LUMEN_EDEVICE, LUMEN_ENOMEM = 0, 1, 2, 3
+
+STAGE = {
+    "cost": 0,
+    "path": 1,
+    "sum": 2,
+    "wta_int": 3,
+    "disp_left": 4,
+    "disp_right": 5,
+    "after_lr": 6,
+    "after_speckle": 7,
+    "output": 8,
+}
+
+
+def middleburyMatcherMaxDisparity(calibNumDisparities: int, fallbackWhenMissing: int = 96) -> int:
+    """include/stereo/sgbm_limits.h:18-22."""
+    ndisp = calibNumDisparities if calibNumDisparities > 0 else fallbackWhenMissing
+    return min(ndisp, kSgbmMaxDisparityRange)
+
+
+class StereoException(RuntimeError):
+    """Invalid input or parameters (stereo::StereoException, include/stereo/common.h:105-110)."""
+
+
+class GpuException(RuntimeError):
+    """Device/runtime failure. Replaces the reference's CudaException
+    (include/stereo/common.h:31-48); ``error_code`` is the library status code."""
+
+    def __init__(self, msg: str, error_code: int = LUMEN_EDEVICE):
+        super().__init__(msg)
+        self.error_code = error_code
+
+
+class MatchingCostMode(enum.IntEnum):
+    """include/stereo/stereo_params.h:20-23."""
+
+    SAD = 0
+    Census = 1
+
+
+@dataclasses.dataclass
+class StereoParams:
+    """include/stereo/stereo_params.h:57-104 (same fields, same defaults)."""
+
+    minDisparity: int = 0
+    maxDisparity: int = 128
+    blockSize: int = 5
+    P1: int = 200
+    P2: int = 800
+    preFilterCap: int = 31
+    uniquenessRatio: int = 10
+    disp12MaxDiff: int = 1
+    speckleWindowSize: int = 100
+    speckleRange: int = 32
+    useGPU: bool = True
+    useSharedMemory: bool = True
+    numDirections: int = 4
+    matchingCostMode: MatchingCostMode = MatchingCostMode.SAD
+
+    def autoComputePenalties(self, numChannels: int = 1) -> None:
+        """include/stereo/stereo_params.h:88-91."""
+        self.P1 = 8 * numChannels * self.blockSize * self.blockSize
+        self.P2 = 32 * numChannels * self.blockSize * self.blockSize
+
+    def isValid(self) -> bool:
+        """include/stereo/stereo_params.h:96-103 (not called by compute, as in the reference)."""
+        if self.blockSize < 1 or self.blockSize % 2 == 0:
+            return False
+        if self.minDisparity < 0:
+            return False
+        if self.maxDisparity <= self.minDisparity:
+            return False
+        if self.P2 <= self.P1:
+            return False
+        if self.uniquenessRatio < 0 or self.uniquenessRatio > 100:
+            return False
+        return True
+
+    def to_c(self) -> "CParams":
+        return CParams(
+            int(self.minDisparity), int(self.maxDisparity), int(self.blockSize), int(self.P1),
+            int(self.P2), int(self.preFilterCap), int(self.uniquenessRatio),
+            int(self.disp12MaxDiff), int(self.speckleWindowSize), int(self.speckleRange),
+            int(bool(self.useGPU)), int(bool(self.useSharedMemory)), int(self.numDirections),
+            int(self.matchingCostMode),
+        )
+
+    @classmethod
+    def from_c(cls, c: "CParams") -> "StereoParams":
+        return cls(
+            c.minDisparity, c.maxDisparity, c.blockSize, c.P1, c.P2, c.preFilterCap,
+            c.uniquenessRatio, c.disp12MaxDiff, c.speckleWindowSize, c.speckleRange,
+            bool(c.useGPU), bool(c.useSharedMemory), c.numDirections,
+            MatchingCostMode(c.matchingCostMode) if c.matchingCostMode in (0, 1) else c.matchingCostMode,
+        )
+
+
+class CParams(ctypes.Structure):
+    """lumen_stereo_params (include/lumen_sgbm.h)."""
+
+    _fields_ = [(name, ctypes.c_int32) for name in (
+        "minDisparity", "maxDisparity", "blockSize", "P1", "P2", "preFilterCap",
+        "uniquenessRatio", "disp12MaxDiff", "speckleWindowSize", "speckleRange", "useGPU",
+        "useSharedMemory", "numDirections", "matchingCostMode")]
+
+
+_lib = None
+
+# Every symbol include/lumen_sgbm.h declares, with its ctypes signature.
+_c_int, _c_void_p, _c_size_t = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
+_SIGNATURES = {
+    "lumen_sgbm_default_params": (_c_int, [ctypes.POINTER(CParams)]),
+    "lumen_sgbm_create": (_c_int, [ctypes.POINTER(CParams), ctypes.POINTER(_c_void_p)]),
+    "lumen_sgbm_destroy": (_c_int, [_c_void_p]),
+    "lumen_sgbm_set_params": (_c_int, [_c_void_p, ctypes.POINTER(CParams)]),
+    "lumen_sgbm_get_params": (_c_int, [_c_void_p, ctypes.POINTER(CParams)]),
+    "lumen_sgbm_compute": (_c_int, [_c_void_p, _c_void_p, _c_size_t, _c_void_p, _c_size_t, _c_int,
+                                    _c_int, _c_int, _c_void_p, _c_size_t]),
+    "lumen_sgbm_compute_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
+                                           _c_void_p, _c_int]),
+    "lumen_sgbm_get_stream": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
+    "lumen_sgbm_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "lumen_sgbm_get_timings_avg": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(ctypes.c_float)]),
+    "lumen_sgbm_last_error": (ctypes.c_char_p, [_c_void_p]),
+    "lumen_last_error": (ctypes.c_char_p, []),
+    "lumen_sgbm_dump_stage": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_size_t]),
+    "lumen_sgbm_num_paths": (_c_int, [_c_void_p]),
+    "lumen_sgbm_footprint": (_c_int, [ctypes.POINTER(CParams), _c_int, _c_int,
+                                      ctypes.POINTER(_c_size_t)]),
+    "lumen_synth_pair": (_c_int, [_c_int, _c_int, ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_uint64, ctypes.c_float, _c_void_p, _c_void_p,
+                                  _c_void_p]),
+}
+
+
+def library() -> ctypes.CDLL:
+    """Load liblumen_sgbm.so (built in-tree by __graft_entry__.build()). Raises if absent."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                "g.build()'` (there is no CPU fallback)")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def synth_pair(width: int, height: int, dmin: float, dmax: float, seed: int,
+               lowtex_frac: float = 0.0, with_gt: bool = False):
+    """Deterministic synthetic pair (see include/lumen_sgbm.h, lumen_synth_pair)."""
+    lib = library()
+    left = np.empty((height, width), np.uint8)
+    right = np.empty((height, width), np.uint8)
+    gt = np.empty((height, width), np.float32) if with_gt else None
+    rc = lib.lumen_synth_pair(width, height, float(dmin), float(dmax), seed & (2**64 - 1),
+                              float(lowtex_frac), _ptr(left), _ptr(right),
+                              _ptr(gt) if gt is not None else None)
+    if rc != LUMEN_OK:
+        raise StereoException("lumen_synth_pair failed")
+    return (left, right, gt) if with_gt else (left, right)
+
+
+class StereoSGBM:
+    """GPU SGBM matcher (include/stereo/sgbm_gpu.h:29-91)."""
+
+    def __init__(self, params: StereoParams | None = None):
+        self._lib = library()
+        self._h = ctypes.c_void_p()
+        p = (params or StereoParams()).to_c()
+        rc = self._lib.lumen_sgbm_create(ctypes.byref(p), ctypes.byref(self._h))
+        if rc != LUMEN_OK:
+            raise GpuException(self._lib.lumen_last_error().decode(), rc)
+        self._times = np.zeros(6, np.float32)
+        self._shape = (0, 0)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.lumen_sgbm_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+    def _raise(self, rc: int):
+        msg = self._lib.lumen_sgbm_last_error(self._h).decode()
+        if rc == LUMEN_EINVAL:
+            raise StereoException(msg)
+        raise GpuException(msg, rc)
+
+    def setParams(self, params: StereoParams) -> None:
+        p = params.to_c()
+        rc = self._lib.lumen_sgbm_set_params(self._h, ctypes.byref(p))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+
+    def getParams(self) -> StereoParams:
+        p = CParams()
+        self._lib.lumen_sgbm_get_params(self._h, ctypes.byref(p))
+        return StereoParams.from_c(p)
+
+    def compute(self, left: np.ndarray, right: np.ndarray) -> np.ndarray:
+        """Disparity (int16, 1/16 px, -1 invalid) of a rectified pair: uint8 HxW grayscale or
+        HxWx3 BGR (src/sgbm_gpu.cu:147-195)."""
+        left = np.asarray(left)
+        right = np.asarray(right)
+        if left.size == 0 or right.size == 0:
+            raise StereoException("Input images are empty")
+        if left.shape != right.shape:
+            raise StereoException("Left and right images must have same size")
+        if left.dtype != np.uint8 or right.dtype != np.uint8:
+            raise StereoException("Input images must be uint8")
+        left = np.ascontiguousarray(left)
+        right = np.ascontiguousarray(right)
+        h, w = left.shape[:2]
+        ch = 1 if left.ndim == 2 else left.shape[2]
+        out = np.empty((h, w), np.int16)
+        rc = self._lib.lumen_sgbm_compute(self._h, _ptr(left), w * ch, _ptr(right), w * ch, w, h,
+                                          ch, _ptr(out), w * 2)
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        self._shape = (h, w)
+        self._lib.lumen_sgbm_get_timings(self._h, self._times.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_float)))
+        return out
+
+    def compute_device(self, d_left: int, d_right: int, width: int, height: int, d_out: int,
+                       sync: bool = True) -> None:
+        """Device-pointer form: inputs and output already in HBM (see lumen_sgbm_compute_device)."""
+        rc = self._lib.lumen_sgbm_compute_device(self._h, d_left, d_right, width, height, d_out,
+                                                 int(sync))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        self._shape = (height, width)
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        rc = self._lib.lumen_sgbm_get_stream(self._h, ctypes.byref(s))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        return s.value or 0
+
+    def timings(self, ncalls: int = 1) -> np.ndarray:
+        """[total, cost, aggregation, post, sum+WTA+LR, upload] in ms (averaged over ncalls)."""
+        t = np.zeros(6, np.float32)
+        fp = t.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        rc = (self._lib.lumen_sgbm_get_timings(self._h, fp) if ncalls == 1
+              else self._lib.lumen_sgbm_get_timings_avg(self._h, ncalls, fp))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        return t
+
+    def dump(self, stage: str, path: int = 0) -> np.ndarray:
+        """One stage of the last compute in the reference's layout (parity hook)."""
+        p = self.getParams()
+        st = STAGE[stage]
+        shape_hw = self._shape
+        D = p.maxDisparity - p.minDisparity
+        if st == 0:
+            a = np.empty(shape_hw + (D,), np.uint8)
+        elif st in (1, 2):
+            a = np.empty(shape_hw + (D,), np.uint16)
+        else:
+            a = np.empty(shape_hw, np.int16)
+        rc = self._lib.lumen_sgbm_dump_stage(self._h, st, path, _ptr(a), a.nbytes)
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        return a
+
+    def numPaths(self) -> int:
+        return self._lib.lumen_sgbm_num_paths(self._h)
+
+    # Timing getters of include/stereo/sgbm_gpu.h:52-55.
+    def getLastComputeTimeMs(self) -> float:
+        return float(self._times[0])
+
+    def getCostComputeTimeMs(self) -> float:
+        return float(self._times[1])
+
+    def getAggregationTimeMs(self) -> float:
+        return float(self._times[2])
+
+    def getPostProcessTimeMs(self) -> float:
+        return float(self._times[3])

@@ -1,0 +1,76 @@
+"""Build recipe for liblumen_sgbm.so (gfx950) and the oracle's CPU library.
+
+Run ``python -m lumenstereo_amd._build`` (or ``__graft_entry__.build()``). Everything is
+compiled in-tree so the shared objects travel with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+PKG = ROOT / "lumenstereo_amd"
+CSRC = PKG / "csrc"
+LIB = PKG / "liblumen_sgbm.so"
+ORACLE_DIR = ROOT / "oracle"
+ORACLE_LIB = ORACLE_DIR / "liboracle.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_capi.hip"]
+CXX_SOURCES = ["synth_scene.cpp"]
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def _run(cmd: list[str]) -> None:
+    print("+", " ".join(str(c) for c in cmd), flush=True)
+    subprocess.run([str(c) for c in cmd], check=True)
+
+
+def build_library() -> Path:
+    inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
+    objdir = PKG / "_objs"
+    objdir.mkdir(exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        _run([HIPCC, *HIP_FLAGS, *inc, "-c", CSRC / src, "-o", obj])
+        objs.append(obj)
+    for src in CXX_SOURCES:
+        obj = objdir / (Path(src).stem + ".o")
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", *inc, "-c", CSRC / src,
+              "-o", obj])
+        objs.append(obj)
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> Path:
+    tmp = ORACLE_LIB.with_suffix(".so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
+          ORACLE_DIR / "sgbm_oracle.cpp", "-o", tmp])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def build_all() -> None:
+    build_oracle()
+    build_library()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,546 @@
+// sgbm_capi.hip - C ABI of the matcher: validation, device buffers, stage order, timing.
+//
+// Stage order follows StereoSGBM::compute (src/sgbm_gpu.cu:147-195):
+//   upload -> cost (census or prefilter+SAD) -> all paths -> sum+WTA(L,R)+LR -> speckle ->
+//   median -> download.
+// Buffers grow on demand and are kept between calls (src/sgbm_gpu.cu:119-145 resizes lazily
+// too). Nothing is zeroed per call: every buffer a stage reads was fully written by the stage
+// before it.
+#include "lumen_sgbm.h"
+#include "sgbm_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace lumen;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && ptr) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        if (bytes == 0) return hipSuccess;
+        hipError_t e = hipMalloc(&ptr, bytes);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = bytes;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+constexpr int kNumEvents = 6;  // start, uploaded, cost, paths, wta, post
+constexpr int kRing = 64;
+
+struct Geometry {
+    int W = 0, H = 0, D = 0, Dp = 0, m = 0, npaths = 0, costScale = 0;
+    size_t pixels() const { return (size_t)W * H; }
+    size_t volume() const { return pixels() * Dp; }
+};
+
+int num_paths(const lumen_stereo_params& p) { return p.numDirections >= 8 ? 8 : 4; }
+
+// src/sgbm_gpu.cu:312-317.
+int cost_scale(const lumen_stereo_params& p) {
+    if (p.matchingCostMode == 0) return p.blockSize * p.blockSize;
+    return std::max(1, (p.blockSize * p.blockSize * 2 * p.preFilterCap) / 255);
+}
+
+}  // namespace
+
+struct lumen_sgbm {
+    lumen_stereo_params params{};
+    std::string err;
+    bool ready = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[kRing][kNumEvents] = {};
+    int ringNext = 0;   // next slot to record
+    int ringCount = 0;  // completed-or-pending calls recorded
+    Geometry geo;
+    bool haveResult = false;
+    DevBuf left, right, censusL, censusR, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
+        sum, scratchOut;
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE;
+    }
+    int ensure_ready() {
+        if (ready) return LUMEN_OK;
+        hipError_t e = hipGetDevice(&device);
+        if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+        e = hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+        for (int i = 0; i < kRing; ++i)
+            for (int j = 0; j < kNumEvents; ++j) {
+                e = hipEventCreate(&ev[i][j]);
+                if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+            }
+        ready = true;
+        return LUMEN_OK;
+    }
+    ~lumen_sgbm() {
+        if (ready) {
+            (void)hipStreamSynchronize(stream);
+            for (int i = 0; i < kRing; ++i)
+                for (int j = 0; j < kNumEvents; ++j)
+                    if (ev[i][j]) (void)hipEventDestroy(ev[i][j]);
+            (void)hipStreamDestroy(stream);
+        }
+        DevBuf* all[] = {&left, &right, &censusL, &censusR, &cost, &paths, &dispL,
+                         &dispR, &dispLR, &wtaInt, &spk, &out, &sum, &scratchOut};
+        for (DevBuf* b : all) b->release();
+    }
+};
+
+namespace {
+
+// Validation of src/sgbm_gpu.cu:148-166, plus the limits of this implementation.
+int validate(lumen_sgbm* h, const void* left, const void* right, int W, int H) {
+    if (!left || !right || W <= 0 || H <= 0) return h->fail(LUMEN_EINVAL, "Input images are empty");
+    const lumen_stereo_params& p = h->params;
+    const int D = p.maxDisparity - p.minDisparity;
+    if (D < 1)
+        return h->fail(LUMEN_EINVAL,
+                       "Invalid disparity range: maxDisparity must be greater than minDisparity");
+    if (D > LUMEN_MAX_DISPARITY_RANGE)
+        return h->fail(LUMEN_EINVAL,
+                       "Disparity range (maxDisparity - minDisparity) exceeds GPU limit (" +
+                           std::to_string(LUMEN_MAX_DISPARITY_RANGE) + "). Reduce the search range.");
+    if (p.minDisparity < 0)
+        return h->fail(LUMEN_EINVAL, "minDisparity must be >= 0");
+    if (p.matchingCostMode != 1 && W < 2)
+        return h->fail(LUMEN_EINVAL, "Image width must be at least 2 for the SAD prefilter");
+    if (W > kMaxWidth)
+        return h->fail(LUMEN_EINVAL, "Image width exceeds " + std::to_string(kMaxWidth));
+    return LUMEN_OK;
+}
+
+int prepare(lumen_sgbm* h, int W, int H) {
+    int rc = h->ensure_ready();
+    if (rc) return rc;
+    Geometry& g = h->geo;
+    const lumen_stereo_params& p = h->params;
+    g.W = W;
+    g.H = H;
+    g.m = p.minDisparity;
+    g.D = p.maxDisparity - p.minDisparity;
+    g.Dp = padded_range(g.D);
+    g.npaths = num_paths(p);
+    g.costScale = cost_scale(p);
+    const size_t n = g.pixels();
+    struct {
+        DevBuf* b;
+        size_t bytes;
+        const char* name;
+    } plan[] = {
+        {&h->left, n, "left image"},
+        {&h->right, n, "right image"},
+        {&h->censusL, p.matchingCostMode == 1 ? n * 4 : 0, "left census"},
+        {&h->censusR, p.matchingCostMode == 1 ? n * 4 : 0, "right census"},
+        {&h->cost, g.volume(), "cost volume"},
+        {&h->paths, g.volume() * 2 * g.npaths, "path costs"},
+        {&h->dispL, n * 2, "left disparity"},
+        {&h->dispR, n * 2, "right disparity"},
+        {&h->dispLR, n * 2, "checked disparity"},
+        {&h->wtaInt, n * 2, "integer disparity"},
+        {&h->spk, n * 2, "speckle output"},
+        {&h->out, n * 2, "output"},
+    };
+    for (auto& e : plan) {
+        if (e.bytes == 0) continue;
+        hipError_t err = e.b->ensure(e.bytes);
+        if (err != hipSuccess) return h->hip_fail(err, e.name);
+    }
+    return LUMEN_OK;
+}
+
+// Enqueues every stage after the upload; images are already in h->left / h->right or in the
+// caller's device buffers.
+int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dout, int slot) {
+    const Geometry& g = h->geo;
+    const lumen_stereo_params& p = h->params;
+    hipStream_t s = h->stream;
+    hipError_t e;
+    if (p.matchingCostMode == 1) {
+        e = launch_census(dl, dr, h->censusL.as<uint32_t>(), h->censusR.as<uint32_t>(), g.W, g.H, s);
+        if (e != hipSuccess) return h->hip_fail(e, "census launch");
+        e = launch_cost_census(h->censusL.as<uint32_t>(), h->censusR.as<uint32_t>(),
+                               h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, s);
+        if (e != hipSuccess) return h->hip_fail(e, "census cost launch");
+    } else {
+        e = launch_cost_sad(dl, dr, h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, p.blockSize,
+                            p.preFilterCap, s);
+        if (e != hipSuccess) return h->hip_fail(e, "SAD cost launch");
+    }
+    e = hipEventRecord(h->ev[slot][2], s);
+    if (e != hipSuccess) return h->hip_fail(e, "event record");
+
+    PathsArgs pa{};
+    pa.cost = h->cost.as<uint8_t>();
+    pa.img = dl;
+    pa.L = h->paths.as<uint16_t>();
+    pa.pathStride = g.volume();
+    pa.W = g.W;
+    pa.H = g.H;
+    pa.D = g.D;
+    pa.Dp = g.Dp;
+    pa.P1 = p.P1;
+    pa.costScale = g.costScale;
+    pa.npaths = g.npaths;
+    // t16(P2adj) for every |dI| (src/sgbm_kernels.cu:304-310).
+    for (int gd = 0; gd < 256; ++gd) {
+        const int p2 = std::max(p.P1 + 1, p.P2 * 8 / std::max(8, gd));
+        pa.p2tab[gd] = static_cast<uint16_t>(static_cast<unsigned>(p2) & 0xFFFFu);
+    }
+    e = launch_paths(pa, s);
+    if (e != hipSuccess) return h->hip_fail(e, "path aggregation launch");
+    e = hipEventRecord(h->ev[slot][3], s);
+    if (e != hipSuccess) return h->hip_fail(e, "event record");
+
+    WtaArgs wa{};
+    wa.L = h->paths.as<uint16_t>();
+    wa.pathStride = g.volume();
+    wa.npaths = g.npaths;
+    wa.W = g.W;
+    wa.H = g.H;
+    wa.D = g.D;
+    wa.Dp = g.Dp;
+    wa.m = g.m;
+    wa.uniquenessRatio = p.uniquenessRatio;
+    wa.doLR = p.disp12MaxDiff >= 0 ? 1 : 0;
+    wa.lrMaxDiff = p.disp12MaxDiff * 16;
+    wa.dispL = h->dispL.as<int16_t>();
+    wa.dispR = h->dispR.as<int16_t>();
+    wa.dispLR = h->dispLR.as<int16_t>();
+    wa.wtaInt = h->wtaInt.as<int16_t>();
+    e = launch_wta(wa, s);
+    if (e != hipSuccess) return h->hip_fail(e, "WTA launch");
+    e = hipEventRecord(h->ev[slot][4], s);
+    if (e != hipSuccess) return h->hip_fail(e, "event record");
+
+    const int16_t* medIn = h->dispLR.as<int16_t>();
+    if (p.speckleWindowSize > 0) {
+        e = launch_speckle(h->dispLR.as<int16_t>(), h->spk.as<int16_t>(), g.W, g.H,
+                           std::min(p.speckleWindowSize, 25), p.speckleRange * 16, s);
+        if (e != hipSuccess) return h->hip_fail(e, "speckle launch");
+        medIn = h->spk.as<int16_t>();
+    }
+    e = launch_median(medIn, dout, g.W, g.H, s);
+    if (e != hipSuccess) return h->hip_fail(e, "median launch");
+    e = hipEventRecord(h->ev[slot][5], s);
+    if (e != hipSuccess) return h->hip_fail(e, "event record");
+    return LUMEN_OK;
+}
+
+int begin_slot(lumen_sgbm* h) {
+    const int slot = h->ringNext;
+    h->ringNext = (h->ringNext + 1) % kRing;
+    h->ringCount = std::min(h->ringCount + 1, kRing);
+    return slot;
+}
+
+// OpenCV's fixed-point BGR->gray (R 0.299, G 0.587, B 0.114 in 14-bit fixed point, rounded),
+// the conversion src/sgbm_gpu.cu:168-175 applies to 3-channel input.
+void bgr_to_gray(const uint8_t* src, size_t stride, int W, int H, uint8_t* dst) {
+    for (int y = 0; y < H; ++y) {
+        const uint8_t* s = src + (size_t)y * stride;
+        for (int x = 0; x < W; ++x) {
+            const int b = s[3 * x], gch = s[3 * x + 1], r = s[3 * x + 2];
+            dst[(size_t)y * W + x] =
+                static_cast<uint8_t>((b * 1868 + gch * 9617 + r * 4899 + (1 << 13)) >> 14);
+        }
+    }
+}
+
+int check_params(const lumen_stereo_params* p) {
+    if (!p) {
+        g_last_error = "params is NULL";
+        return LUMEN_EINVAL;
+    }
+    return LUMEN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lumen_sgbm_default_params(lumen_stereo_params* out) {
+    if (!out) return LUMEN_EINVAL;
+    out->minDisparity = 0;
+    out->maxDisparity = 128;
+    out->blockSize = 5;
+    out->P1 = 200;
+    out->P2 = 800;
+    out->preFilterCap = 31;
+    out->uniquenessRatio = 10;
+    out->disp12MaxDiff = 1;
+    out->speckleWindowSize = 100;
+    out->speckleRange = 32;
+    out->useGPU = 1;
+    out->useSharedMemory = 1;
+    out->numDirections = 4;
+    out->matchingCostMode = 0;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_create(const lumen_stereo_params* params, lumen_sgbm** out) {
+    if (!out) {
+        g_last_error = "out is NULL";
+        return LUMEN_EINVAL;
+    }
+    *out = nullptr;
+    lumen_stereo_params p;
+    if (params)
+        p = *params;
+    else
+        lumen_sgbm_default_params(&p);
+    lumen_sgbm* h = new (std::nothrow) lumen_sgbm();
+    if (!h) {
+        g_last_error = "out of host memory";
+        return LUMEN_ENOMEM;
+    }
+    h->params = p;
+    *out = h;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_destroy(lumen_sgbm* h) {
+    delete h;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_set_params(lumen_sgbm* h, const lumen_stereo_params* params) {
+    if (!h) return LUMEN_EINVAL;
+    if (check_params(params)) return h->fail(LUMEN_EINVAL, "params is NULL");
+    h->params = *params;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_get_params(const lumen_sgbm* h, lumen_stereo_params* out) {
+    if (!h || !out) return LUMEN_EINVAL;
+    *out = h->params;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_num_paths(const lumen_sgbm* h) { return h ? num_paths(h->params) : 0; }
+
+int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int height,
+                         size_t* bytes_out) {
+    if (!params || !bytes_out || width <= 0 || height <= 0) return LUMEN_EINVAL;
+    const int D = params->maxDisparity - params->minDisparity;
+    if (D < 1 || D > LUMEN_MAX_DISPARITY_RANGE) return LUMEN_EINVAL;
+    const size_t n = (size_t)width * height;
+    const size_t vol = n * padded_range(D);
+    size_t b = vol + vol * 2 * num_paths(*params) + n * 2 + n * 2 * 6;
+    if (params->matchingCostMode == 1) b += n * 8;
+    *bytes_out = b;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
+                       const uint8_t* right, size_t right_stride, int width, int height,
+                       int channels, int16_t* out, size_t out_stride) {
+    if (!h) return LUMEN_EINVAL;
+    h->err.clear();
+    int rc = validate(h, left, right, width, height);
+    if (rc) return rc;
+    if (!out) return h->fail(LUMEN_EINVAL, "Output buffer is NULL");
+    rc = prepare(h, width, height);
+    if (rc) return rc;
+    h->haveResult = false;
+    const int W = width, H = height;
+    const int ch = channels == 3 ? 3 : 1;
+    if (left_stride == 0) left_stride = (size_t)W * ch;
+    if (right_stride == 0) right_stride = (size_t)W * ch;
+    if (out_stride == 0) out_stride = (size_t)W * 2;
+    std::vector<uint8_t> grayL, grayR;
+    if (ch == 3) {
+        grayL.resize((size_t)W * H);
+        grayR.resize((size_t)W * H);
+        bgr_to_gray(left, left_stride, W, H, grayL.data());
+        bgr_to_gray(right, right_stride, W, H, grayR.data());
+        left = grayL.data();
+        right = grayR.data();
+        left_stride = right_stride = (size_t)W;
+    }
+    const int slot = begin_slot(h);
+    hipStream_t s = h->stream;
+    hipError_t e = hipEventRecord(h->ev[slot][0], s);
+    if (e != hipSuccess) return h->hip_fail(e, "event record");
+    e = hipMemcpy2DAsync(h->left.ptr, W, left, left_stride, W, H, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return h->hip_fail(e, "upload left");
+    e = hipMemcpy2DAsync(h->right.ptr, W, right, right_stride, W, H, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return h->hip_fail(e, "upload right");
+    e = hipEventRecord(h->ev[slot][1], s);
+    if (e != hipSuccess) return h->hip_fail(e, "event record");
+    rc = run_stages(h, h->left.as<uint8_t>(), h->right.as<uint8_t>(), h->out.as<int16_t>(), slot);
+    if (rc) return rc;
+    e = hipMemcpy2DAsync(out, out_stride, h->out.ptr, (size_t)W * 2, (size_t)W * 2, H,
+                         hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return h->hip_fail(e, "download");
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return h->hip_fail(e, "compute");
+    h->haveResult = true;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_compute_device(lumen_sgbm* h, const uint8_t* d_left, const uint8_t* d_right,
+                              int width, int height, int16_t* d_out, int sync) {
+    if (!h) return LUMEN_EINVAL;
+    h->err.clear();
+    int rc = validate(h, d_left, d_right, width, height);
+    if (rc) return rc;
+    if (!d_out) return h->fail(LUMEN_EINVAL, "Output buffer is NULL");
+    rc = prepare(h, width, height);
+    if (rc) return rc;
+    h->haveResult = false;
+    const int slot = begin_slot(h);
+    hipStream_t s = h->stream;
+    for (int k = 0; k < 2; ++k) {
+        hipError_t e = hipEventRecord(h->ev[slot][k], s);
+        if (e != hipSuccess) return h->hip_fail(e, "event record");
+    }
+    rc = run_stages(h, d_left, d_right, d_out, slot);
+    if (rc) return rc;
+    // The dump hook reads the output from the handle's own buffer.
+    if (d_out != h->out.as<int16_t>()) {
+        hipError_t e = hipMemcpyAsync(h->out.ptr, d_out, h->geo.pixels() * 2,
+                                      hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return h->hip_fail(e, "output copy");
+    }
+    if (sync) {
+        hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return h->hip_fail(e, "compute");
+    }
+    h->haveResult = true;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_get_stream(lumen_sgbm* h, void** stream_out) {
+    if (!h || !stream_out) return LUMEN_EINVAL;
+    int rc = h->ensure_ready();
+    if (rc) return rc;
+    *stream_out = static_cast<void*>(h->stream);
+    return LUMEN_OK;
+}
+
+static int read_slot(lumen_sgbm* h, int slot, float out[6]) {
+    hipError_t e = hipEventSynchronize(h->ev[slot][5]);
+    if (e != hipSuccess) return h->hip_fail(e, "event sync");
+    const int pairs[6][2] = {{0, 5}, {1, 2}, {2, 3}, {4, 5}, {3, 4}, {0, 1}};
+    for (int i = 0; i < 6; ++i) {
+        float ms = 0.0f;
+        e = hipEventElapsedTime(&ms, h->ev[slot][pairs[i][0]], h->ev[slot][pairs[i][1]]);
+        if (e != hipSuccess) return h->hip_fail(e, "event elapsed");
+        out[i] = ms;
+    }
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_get_timings(lumen_sgbm* h, float out[6]) {
+    if (!h || !out) return LUMEN_EINVAL;
+    if (!h->ready || h->ringCount == 0) {
+        for (int i = 0; i < 6; ++i) out[i] = 0.0f;
+        return LUMEN_OK;
+    }
+    const int slot = (h->ringNext + kRing - 1) % kRing;
+    return read_slot(h, slot, out);
+}
+
+int lumen_sgbm_get_timings_avg(lumen_sgbm* h, int ncalls, float out[6]) {
+    if (!h || !out || ncalls < 1) return LUMEN_EINVAL;
+    for (int i = 0; i < 6; ++i) out[i] = 0.0f;
+    if (!h->ready || h->ringCount == 0) return LUMEN_OK;
+    const int n = std::min(ncalls, h->ringCount);
+    for (int k = 1; k <= n; ++k) {
+        float t[6];
+        int rc = read_slot(h, (h->ringNext + kRing - k) % kRing, t);
+        if (rc) return rc;
+        for (int i = 0; i < 6; ++i) out[i] += t[i] / n;
+    }
+    return LUMEN_OK;
+}
+
+const char* lumen_sgbm_last_error(const lumen_sgbm* h) { return h ? h->err.c_str() : ""; }
+
+const char* lumen_last_error(void) { return g_last_error.c_str(); }
+
+int lumen_sgbm_dump_stage(lumen_sgbm* h, int stage, int path, void* host, size_t bytes) {
+    if (!h || !host) return LUMEN_EINVAL;
+    if (!h->haveResult) return h->fail(LUMEN_EINVAL, "no completed compute to dump");
+    const Geometry& g = h->geo;
+    const size_t n = g.pixels();
+    hipStream_t s = h->stream;
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return h->hip_fail(e, "sync");
+    auto strip = [&](const void* dev, size_t elem) -> int {
+        // Padded [pixels][Dp] -> host [pixels][D].
+        const size_t need = n * g.D * elem;
+        if (bytes < need) return h->fail(LUMEN_EINVAL, "dump buffer too small");
+        hipError_t err = hipMemcpy2D(host, g.D * elem, dev, g.Dp * elem, g.D * elem, n,
+                                     hipMemcpyDeviceToHost);
+        if (err != hipSuccess) return h->hip_fail(err, "dump copy");
+        return LUMEN_OK;
+    };
+    auto plain = [&](const void* dev) -> int {
+        if (bytes < n * 2) return h->fail(LUMEN_EINVAL, "dump buffer too small");
+        hipError_t err = hipMemcpy(host, dev, n * 2, hipMemcpyDeviceToHost);
+        if (err != hipSuccess) return h->hip_fail(err, "dump copy");
+        return LUMEN_OK;
+    };
+    switch (stage) {
+        case LUMEN_STAGE_COST: return strip(h->cost.ptr, 1);
+        case LUMEN_STAGE_PATH:
+            if (path < 0 || path >= g.npaths) return h->fail(LUMEN_EINVAL, "path index out of range");
+            return strip(h->paths.as<uint16_t>() + (size_t)path * g.volume(), 2);
+        case LUMEN_STAGE_SUM: {
+            e = h->sum.ensure(g.volume() * 2);
+            if (e != hipSuccess) return h->hip_fail(e, "sum buffer");
+            e = launch_path_sum(h->paths.as<uint16_t>(), g.volume(), g.npaths, h->sum.as<uint16_t>(),
+                                g.volume(), s);
+            if (e != hipSuccess) return h->hip_fail(e, "sum launch");
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return h->hip_fail(e, "sum");
+            return strip(h->sum.ptr, 2);
+        }
+        case LUMEN_STAGE_WTA_INT: return plain(h->wtaInt.ptr);
+        case LUMEN_STAGE_DISP_LEFT: return plain(h->dispL.ptr);
+        case LUMEN_STAGE_DISP_RIGHT: return plain(h->dispR.ptr);
+        case LUMEN_STAGE_AFTER_LR: return plain(h->dispLR.ptr);
+        case LUMEN_STAGE_AFTER_SPECKLE:
+            return plain(h->params.speckleWindowSize > 0 ? h->spk.ptr : h->dispLR.ptr);
+        case LUMEN_STAGE_OUTPUT: return plain(h->out.ptr);
+        default: return h->fail(LUMEN_EINVAL, "unknown stage");
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+"""Stage-by-stage parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Every stage is compared bit-exactly: cost volume, each path's aggregated costs, their sum,
+integer WTA, left/right subpixel maps, after the LR check, after speckle, final output.
+"""
+import numpy as np
+import pytest
+
+import lumenstereo_amd as ls
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # (W, H, minD, maxD, mode, paths, extra params)
+    (96, 64, 0, 32, 0, 4, {}),
+    (96, 64, 0, 32, 1, 8, {}),
+    (130, 77, 0, 40, 0, 8, {}),          # padded D (40 -> 64), odd sizes
+    (130, 77, 3, 43, 1, 4, {}),          # minDisparity > 0
+    (160, 90, 0, 64, 1, 8, {"uniquenessRatio": 10}),
+    (160, 90, 0, 64, 0, 4, {"disp12MaxDiff": -1}),
+    (120, 80, 0, 96, 0, 4, {"blockSize": 3}),
+    (120, 80, 0, 48, 0, 4, {"blockSize": 7, "preFilterCap": 15}),
+    (100, 70, 0, 32, 1, 4, {"P1": 30000, "P2": 60000}),   # forces uint16 wrap
+    (100, 70, 0, 32, 0, 8, {"speckleWindowSize": 20, "speckleRange": 2}),
+    (200, 40, 0, 130, 1, 8, {}),          # D > W/2, padded to 192
+    (64, 200, 0, 16, 0, 8, {}),           # tall image, short diagonals
+]
+
+
+def _params(minD, maxD, mode, paths, extra):
+    p = ls.StereoParams(minDisparity=minD, maxDisparity=maxD, numDirections=paths,
+                        matchingCostMode=ls.MatchingCostMode(mode), uniquenessRatio=0,
+                        speckleWindowSize=0)
+    for k, v in extra.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}x{c[1]}_d{c[2]}-{c[3]}_m{c[4]}_p{c[5]}_{len(c[6])}")
+def test_stage_parity(gpu_available, case):
+    W, H, minD, maxD, mode, paths, extra = case
+    left, right = ls.synth_pair(W, H, max(minD, 2), maxD - 4, seed=W * 1000 + H)
+    p = _params(minD, maxD, mode, paths, extra)
+    want, st = oracle.sgbm(p, left, right, dumps=True)
+    m = ls.StereoSGBM(p)
+    got = m.compute(left, right)
+    np.testing.assert_array_equal(m.dump("cost"), st["cost"], err_msg="cost volume")
+    for r in range(m.numPaths()):
+        np.testing.assert_array_equal(m.dump("path", r), st["paths"][r],
+                                      err_msg=f"path {oracle.PATH_NAMES[r]}")
+    np.testing.assert_array_equal(m.dump("sum"), st["sum"], err_msg="path sum")
+    np.testing.assert_array_equal(m.dump("wta_int"), st["wta_int"], err_msg="integer WTA")
+    np.testing.assert_array_equal(m.dump("disp_left"), st["disp_left"], err_msg="left WTA")
+    np.testing.assert_array_equal(m.dump("disp_right"), st["disp_right"], err_msg="right WTA")
+    np.testing.assert_array_equal(m.dump("after_lr"), st["after_lr"], err_msg="LR check")
+    np.testing.assert_array_equal(m.dump("after_speckle"), st["after_speckle"], err_msg="speckle")
+    np.testing.assert_array_equal(got, want, err_msg="output")
+
+
+def test_repeat_and_reuse(gpu_available):
+    """Buffers are reused across calls of different sizes; results must not depend on history."""
+    p = _params(0, 48, 1, 8, {})
+    m = ls.StereoSGBM(p)
+    for (W, H) in [(128, 96), (64, 48), (128, 96), (150, 50)]:
+        left, right = ls.synth_pair(W, H, 2, 40, seed=W + H)
+        got = m.compute(left, right)
+        np.testing.assert_array_equal(got, oracle.sgbm(p, left, right))
+
+
+def test_bgr_input(gpu_available):
+    rng = np.random.default_rng(3)
+    left, right = ls.synth_pair(96, 64, 2, 28, seed=11)
+    lc = np.stack([left, np.roll(left, 1, 1), left[::-1, ::-1]], axis=2).copy()
+    rc = np.stack([right, np.roll(right, 1, 1), right[::-1, ::-1]], axis=2).copy()
+    p = _params(0, 32, 0, 4, {})
+    got = ls.StereoSGBM(p).compute(lc, rc)
+    assert got.dtype == np.int16 and got.shape == (64, 96)
+    # OpenCV's fixed-point BGR->gray, written out here for the check.
+    gl = ((lc[..., 0].astype(np.int32) * 1868 + lc[..., 1].astype(np.int32) * 9617 +
+           lc[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
+    gr = ((rc[..., 0].astype(np.int32) * 1868 + rc[..., 1].astype(np.int32) * 9617 +
+           rc[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
+    np.testing.assert_array_equal(got, oracle.sgbm(p, gl, gr))
+    del rng
