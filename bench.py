@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the SGBM matcher on one MI355X.
+
+Workload: one 1920x1080 uint8 stereo pair, D=128 (minDisparity 0), Census 5x5 cost, 8 SGM
+paths, P1=200, P2=800 with adaptive P2, WTA + 1/16-px parabola, LR check (1 px), 3x3 median,
+uniqueness 0, speckle off. Synthetic planes-plus-occluders scene (disparities in [4,124]),
+generated deterministically on the host and uploaded once: the timed region starts with the
+inputs resident in HBM. A step is one full compute of the pair.
+
+value = MDE/s = W*H*D * steps / timed seconds / 1e6 (whole job; one GPU).
+roofline = the dominant kernel (all-paths SGM aggregation): algorithmic bytes per launch =
+W*H*D * 3 * paths (read the uint8 cost once per path, write each uint16 path cost once), divided
+by its mean launch time from hipEvents recorded around it on the stream it runs on.
+cpu_baseline = the oracle's algorithm (oracle/sgbm_oracle.cpp, paths on 8 threads) on the same
+frame; OpenCV is not available on this image, so the OpenCV StereoSGBM baseline cannot run.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MDE/s + frames/s at 1080p D=128 Census; batched pairs/s and p50/p99 vs plain"
+
+W, H, MIN_D, MAX_D = 1920, 1080, 0, 128
+SEED = 20240611
+
+
+def workload_params(ls):
+    return ls.StereoParams(minDisparity=MIN_D, maxDisparity=MAX_D, blockSize=5, P1=200, P2=800,
+                           preFilterCap=31, uniquenessRatio=0, disp12MaxDiff=1,
+                           speckleWindowSize=0, numDirections=8,
+                           matchingCostMode=ls.MatchingCostMode.Census)
+
+
+def load_traffic():
+    """HBM bytes per aggregation launch measured by rocprofv3 --pmc for this workload
+    (profiles/sgm_paths_1080p_d128_census8_pmc.json), or None."""
+    f = ROOT / "profiles" / "sgm_paths_1080p_d128_census8_pmc.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get("hbm_bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(ls, params, left, right, gpu_out, threads):
+    import oracle
+
+    t0 = time.perf_counter()
+    ref = oracle.sgbm_mt(params, left, right, threads)
+    dt = time.perf_counter() - t0
+    de = W * H * (MAX_D - MIN_D)
+    mismatches = int(np.count_nonzero(ref != gpu_out))
+    return {
+        "value": round(de / dt / 1e6, 3),
+        "unit": "MDE/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one full {W}x{H} D={MAX_D} Census 8-path frame, oracle algorithm with the "
+                  f"paths on {threads} threads ({dt:.2f} s); OpenCV StereoSGBM unavailable",
+        "seconds": round(dt, 3),
+    }, mismatches
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the matcher "
+              "runs one pair per GPU and this build has no multi-GPU path)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    import lumenstereo_amd as ls
+
+    params = workload_params(ls)
+    left, right = ls.synth_pair(W, H, 4, 124, SEED)
+    dev = torch.device("cuda:0")
+    d_left = torch.from_numpy(left).to(dev)
+    d_right = torch.from_numpy(right).to(dev)
+    d_out = torch.empty((H, W), dtype=torch.int16, device=dev)
+    m = ls.StereoSGBM(params)
+    npaths = m.numPaths()
+
+    for _ in range(args.warmup):
+        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stage = m.timings(ncalls=min(args.steps, 64))  # total, cost, agg, post, wta, upload
+
+    D = MAX_D - MIN_D
+    de = W * H * D
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = de * args.steps / elapsed / 1e6
+    agg_ms = float(stage[2])
+    agg_bytes = de * 3 * npaths
+    achieved = agg_bytes / (agg_ms * 1e-3) / 1e9
+    frame_bytes = de * (1 + 5 * npaths)
+
+    gpu_out = d_out.cpu().numpy()
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "MDE/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "uint16",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain single pair {W}x{H} uint8, D={D}, Census 5x5, {npaths} paths, "
+                        "P1=200 P2=800 adaptive, WTA+subpixel, LR check, 3x3 median",
+            "width": W, "height": H, "disparities": D, "cost": "census", "paths": npaths,
+            "seed": SEED,
+        },
+        "frames_per_s": round(1e3 / ms_per_step, 2),
+        "stage_ms": {"cost": round(float(stage[1]), 4), "aggregation": round(agg_ms, 4),
+                     "sum_wta_lr": round(float(stage[4]), 4), "post": round(float(stage[3]), 4),
+                     "pipeline": round(float(stage[0]), 4)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "sgm_paths_kernel (all paths, one launch)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": load_traffic(),
+            "algorithmic_bytes_per_launch": agg_bytes,
+        },
+        "frame_roofline_frac": round(frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "reference_published": {"value": 664, "unit": "MDE/s",
+                                "note": "README: 1920x1080 D=128 in 400 ms on an RTX 4050 Laptop "
+                                        "GPU; cost mode and path count not stated"},
+    }
+    if not args.no_cpu_baseline:
+        cb, mism = cpu_baseline(ls, params, left, right, gpu_out, args.cpu_threads)
+        line["cpu_baseline"] = cb
+        line["parity_vs_oracle_mismatched_pixels"] = mism
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
