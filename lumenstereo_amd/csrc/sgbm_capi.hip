@@ -83,7 +83,7 @@ struct lumen_sgbm {
     Geometry geo;
     bool haveResult = false;
     DevBuf left, right, censusL, censusR, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
-        sum, scratchOut;
+        sum, trash;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -116,7 +116,7 @@ struct lumen_sgbm {
             (void)hipStreamDestroy(stream);
         }
         DevBuf* all[] = {&left, &right, &censusL, &censusR, &cost, &paths, &dispL,
-                         &dispR, &dispLR, &wtaInt, &spk, &out, &sum, &scratchOut};
+                         &dispR, &dispLR, &wtaInt, &spk, &out, &sum, &trash};
         for (DevBuf* b : all) b->release();
     }
 };
@@ -174,6 +174,7 @@ int prepare(lumen_sgbm* h, int W, int H) {
         {&h->wtaInt, n * 2, "integer disparity"},
         {&h->spk, n * 2, "speckle output"},
         {&h->out, n * 2, "output"},
+        {&h->trash, kTrashElems * 2, "trash"},
     };
     for (auto& e : plan) {
         if (e.bytes == 0) continue;
@@ -208,6 +209,7 @@ int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dou
     pa.cost = h->cost.as<uint8_t>();
     pa.img = dl;
     pa.L = h->paths.as<uint16_t>();
+    pa.trash = h->trash.as<uint16_t>();
     pa.pathStride = g.volume();
     pa.W = g.W;
     pa.H = g.H;

@@ -13,6 +13,9 @@ constexpr int kMaxDisparityRange = 512;
 // The fused sum/WTA kernel keeps one row of right-WTA state in LDS: 12 bytes per pixel.
 constexpr int kMaxWidth = 5376;
 
+// Elements of the aggregation kernel's trash buffer (64 lanes x 32 entries).
+constexpr int kTrashElems = 64 * 32;
+
 // Disparity-innermost layouts are padded to a multiple of 64 entries (16 lanes x 4).
 inline int padded_range(int D) { return (D + 63) / 64 * 64; }
 
@@ -20,6 +23,7 @@ struct PathsArgs {
     const uint8_t* cost;   // [H][W][Dp] uint8
     const uint8_t* img;    // left raw image [H][W]
     uint16_t* L;           // [npaths][H][W][Dp] uint16
+    uint16_t* trash;       // kTrashElems uint16: sink for stores past the end of a scanline
     size_t pathStride;     // H*W*Dp
     int W, H, D, Dp;
     int P1, costScale;

@@ -49,6 +49,29 @@ __device__ __forceinline__ uint32_t row_min(uint32_t v) {
     return v;
 }
 
+// Value of lane u of the lane's DPP row (row_newbcast, gfx90a and later); u is folded to a
+// constant after unrolling.
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v, int u) {
+    switch (u) {
+        case 0: return dpp<0x150>(v, v);
+        case 1: return dpp<0x151>(v, v);
+        case 2: return dpp<0x152>(v, v);
+        case 3: return dpp<0x153>(v, v);
+        case 4: return dpp<0x154>(v, v);
+        case 5: return dpp<0x155>(v, v);
+        case 6: return dpp<0x156>(v, v);
+        case 7: return dpp<0x157>(v, v);
+        case 8: return dpp<0x158>(v, v);
+        case 9: return dpp<0x159>(v, v);
+        case 10: return dpp<0x15A>(v, v);
+        case 11: return dpp<0x15B>(v, v);
+        case 12: return dpp<0x15C>(v, v);
+        case 13: return dpp<0x15D>(v, v);
+        case 14: return dpp<0x15E>(v, v);
+        default: return dpp<0x15F>(v, v);
+    }
+}
+
 // float -> int16 as the reference's device conversion: truncate, clamp to the int16 range.
 __device__ __forceinline__ int16_t f2i16(float v) {
     float t = truncf(v);
@@ -454,57 +477,97 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PathsArgs a) {
     }
 
     // Origin: L = t16(C * costScale).
-    int p = p0;
+    const long long cstep = (long long)step * Dp;  // bytes between consecutive cost vectors
+    const uint8_t* lp = cbase + (size_t)p0 * Dp;   // cost pointer of the last issued load
+    int lt = 0;                                     // step index of the last issued load
+    uint16_t* sp = Lbase + (size_t)p0 * Dp;         // path-cost pointer of the current step
+    uint16_t* const trash = a.trash + lane * K;     // stores of finished rows land here
     u16x2 Lv[K / 2];
     {
-        const CostBytes<K> c0 = load_cost<K>(cbase + (size_t)p * Dp);
+        const CostBytes<K> c0 = load_cost<K>(lp);
         expand_cost<K>(c0, cs, Lv);
         if constexpr (PAD) {
 #pragma unroll
             for (int j = 0; j < K / 2; ++j) Lv[j] = as16(bits(Lv[j]) | padm[j]);
         }
-        if (len > 0) store_path<K>(Lbase + (size_t)p * Dp, Lv);
+        store_path<K>(len > 0 ? sp : trash, Lv);
     }
     uint32_t M = row_min(slice_min<K>(Lv));
-    int prevI = a.img[p];
-    int pn = len > 1 ? p + step : p;
-    CostBytes<K> cn = load_cost<K>(cbase + (size_t)pn * Dp);
-    int In = a.img[pn];
 
-    for (int t = 1; t < maxLen; ++t) {
-        const bool act = t < len;
-        const CostBytes<K> cc = cn;
-        const int Ic = In;
-        const int pc = pn;
-        pn = t + 1 < len ? pc + step : pc;
-        cn = load_cost<K>(cbase + (size_t)pn * Dp);
-        In = a.img[pn];
-
-        const int gd = abs(Ic - prevI);
-        prevI = Ic;
-        const u16x2 c3 = splat(M + p2[gd]);
-        const u16x2 Mp = splat(M);
-        u16x2 Ch[K / 2];
-        expand_cost<K>(cc, cs, Ch);
-        uint32_t dm1[K / 2], dp1[K / 2];
-        neighbours<K>(Lv, dm1, dp1);
+    // Two register blocks of PF steps alternate: while one block is computed, the costs of the
+    // next are in flight. Lane l of a row also fetches the two image bytes of step
+    // tb + (l % PF) and looks up t16(P2adj) once per block; each step broadcasts its lane's
+    // value with DPP row_newbcast. Loads past the end of a scanline re-read its last pixel;
+    // steps past the end are computed but stored to the trash slot, so every store is
+    // unconditional and the wait counts stay exact.
+    constexpr int PF = K <= 8 ? 8 : (K <= 16 ? 4 : 2);
+    const int last = len > 0 ? len - 1 : 0;
+    const int lu = l & (PF - 1);
+    auto fetch = [&](int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
+        const int ti = min(tb + lu, last);
+        iv[0] = a.img[p0 + ti * step];
+        iv[1] = a.img[p0 + max(ti - 1, 0) * step];
 #pragma unroll
-        for (int j = 0; j < K / 2; ++j) {
-            uint32_t c1 = bits(as16(dm1[j]) + P1p);
-            uint32_t c2 = bits(as16(dp1[j]) + P1p);
-            if (j == 0) c1 |= m0;
-            if constexpr (PAD) {
-                c2 |= c2m[j];
-            } else {
-                if (j == K / 2 - 1) c2 |= c2m[j];
-            }
-            const u16x2 mn = vmin(vmin(Lv[j], as16(c1)), vmin(as16(c2), c3));
-            u16x2 nv = Ch[j] + mn - Mp;
-            if constexpr (PAD) nv = as16(bits(nv) | padm[j]);
-            Lv[j] = nv;
+        for (int u = 0; u < PF; ++u) {
+            const bool adv = lt < last;
+            lp = adv ? lp + cstep : lp;
+            lt += adv ? 1 : 0;
+            dst[u] = load_cost<K>(lp);
         }
-        if (act) store_path<K>(Lbase + (size_t)pc * Dp, Lv);
-        M = row_min(slice_min<K>(Lv));
+    };
+    auto run = [&](int tb, const CostBytes<K>(&blk)[PF], const int (&iv)[2]) {
+        const uint32_t p2v = p2[abs(iv[0] - iv[1])];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int t = tb + u;
+            const bool act = t < len;
+            sp += step * Dp;
+            const u16x2 c3 = splat(M + row_bcast(p2v, u));
+            const u16x2 Mp = splat(M);
+            u16x2 Ch[K / 2];
+            expand_cost<K>(blk[u], cs, Ch);
+            uint32_t dm1[K / 2], dp1[K / 2];
+            neighbours<K>(Lv, dm1, dp1);
+#pragma unroll
+            for (int j = 0; j < K / 2; ++j) {
+                uint32_t c1 = bits(as16(dm1[j]) + P1p);
+                uint32_t c2 = bits(as16(dp1[j]) + P1p);
+                if (j == 0) c1 |= m0;
+                if constexpr (PAD) {
+                    c2 |= c2m[j];
+                } else {
+                    if (j == K / 2 - 1) c2 |= c2m[j];
+                }
+                const u16x2 mn = vmin(vmin(Lv[j], as16(c1)), vmin(as16(c2), c3));
+                u16x2 nv = Ch[j] + mn - Mp;
+                if constexpr (PAD) nv = as16(bits(nv) | padm[j]);
+                Lv[j] = nv;
+            }
+            store_path<K>(act ? sp : trash, Lv);
+            M = row_min(slice_min<K>(Lv));
+        }
+    };
+    CostBytes<K> bufA[PF], bufB[PF];
+    int gA[2], gB[2];
+    fetch(1, bufA, gA);
+    // Entering the loop with block A complete keeps the loop-header state identical on both
+    // edges, so no full drain is needed there.
+    __builtin_amdgcn_s_waitcnt((0 & 0xF) | (7 << 4) | (15 << 8));
+    for (int t0 = 1; t0 < maxLen; t0 += 2 * PF) {
+        // The scheduling barriers keep each block's loads where they are issued: moved next to
+        // their first use they would stall there.
+        fetch(t0 + PF, bufB, gB);
+        __builtin_amdgcn_sched_barrier(0);
+        run(t0, bufA, gA);
+        fetch(t0 + 2 * PF, bufA, gA);
+        __builtin_amdgcn_sched_barrier(0);
+        run(t0 + PF, bufB, gB);
+        // Block A's loads were issued before block B's stores: waiting until only those
+        // stores may be outstanding retires the loads without waiting on the stores.
+        // (s_waitcnt vmcnt(N) expcnt(7) lgkmcnt(15), GFX9 encoding.)
+        constexpr int N = PF * (K % 8 == 0 ? K / 8 : K / 4);
+        static_assert(N < 64, "vmcnt is 6 bits");
+        __builtin_amdgcn_s_waitcnt((N & 0xF) | (7 << 4) | (15 << 8) | (((N >> 4) & 3) << 14));
     }
 }
 
