@@ -66,9 +66,23 @@ def build_oracle() -> Path:
     return ORACLE_LIB
 
 
+CPP_TEST_SRC = ROOT / "tests" / "cpp" / "test_stereo_api.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "test_stereo_api"
+
+
+def build_cpp_tests() -> Path:
+    """The C++ API test program (header-only stereo::StereoSGBM over liblumen_sgbm.so)."""
+    tmp = CPP_TEST_BIN.with_suffix(".tmp")
+    _run(["g++", "-O1", "-std=c++17", "-I", ROOT / "include", CPP_TEST_SRC, "-L", PKG,
+          "-llumen_sgbm", "-Wl,-rpath,$ORIGIN/../../lumenstereo_amd", "-o", tmp])
+    os.replace(tmp, CPP_TEST_BIN)
+    return CPP_TEST_BIN
+
+
 def build_all() -> None:
     build_oracle()
     build_library()
+    build_cpp_tests()
 
 
 if __name__ == "__main__":
