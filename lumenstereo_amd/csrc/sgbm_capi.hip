@@ -82,7 +82,7 @@ struct lumen_sgbm {
     int ringCount = 0;  // completed-or-pending calls recorded
     Geometry geo;
     bool haveResult = false;
-    DevBuf left, right, censusL, censusR, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
+    DevBuf left, right, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
         sum, trash;
 
     int fail(int code, const std::string& msg) {
@@ -115,7 +115,7 @@ struct lumen_sgbm {
                     if (ev[i][j]) (void)hipEventDestroy(ev[i][j]);
             (void)hipStreamDestroy(stream);
         }
-        DevBuf* all[] = {&left, &right, &censusL, &censusR, &cost, &paths, &dispL,
+        DevBuf* all[] = {&left, &right, &cost, &paths, &dispL,
                          &dispR, &dispLR, &wtaInt, &spk, &out, &sum, &trash};
         for (DevBuf* b : all) b->release();
     }
@@ -164,8 +164,6 @@ int prepare(lumen_sgbm* h, int W, int H) {
     } plan[] = {
         {&h->left, n, "left image"},
         {&h->right, n, "right image"},
-        {&h->censusL, p.matchingCostMode == 1 ? n * 4 : 0, "left census"},
-        {&h->censusR, p.matchingCostMode == 1 ? n * 4 : 0, "right census"},
         {&h->cost, g.volume(), "cost volume"},
         {&h->paths, g.volume() * 2 * g.npaths, "path costs"},
         {&h->dispL, n * 2, "left disparity"},
@@ -192,10 +190,7 @@ int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dou
     hipStream_t s = h->stream;
     hipError_t e;
     if (p.matchingCostMode == 1) {
-        e = launch_census(dl, dr, h->censusL.as<uint32_t>(), h->censusR.as<uint32_t>(), g.W, g.H, s);
-        if (e != hipSuccess) return h->hip_fail(e, "census launch");
-        e = launch_cost_census(h->censusL.as<uint32_t>(), h->censusR.as<uint32_t>(),
-                               h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, s);
+        e = launch_cost_census(dl, dr, h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, s);
         if (e != hipSuccess) return h->hip_fail(e, "census cost launch");
     } else {
         e = launch_cost_sad(dl, dr, h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, p.blockSize,
@@ -363,7 +358,6 @@ int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int heigh
     const size_t n = (size_t)width * height;
     const size_t vol = n * padded_range(D);
     size_t b = vol + vol * 2 * num_paths(*params) + n * 2 + n * 2 * 6;
-    if (params->matchingCostMode == 1) b += n * 8;
     *bytes_out = b;
     return LUMEN_OK;
 }

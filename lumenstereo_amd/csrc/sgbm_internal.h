@@ -49,10 +49,8 @@ struct WtaArgs {
 // Number of paths and waves the aggregation grid needs.
 int scanline_count(int path, int W, int H);
 
-hipError_t launch_census(const uint8_t* left, const uint8_t* right, uint32_t* cl, uint32_t* cr,
-                         int W, int H, hipStream_t s);
-hipError_t launch_cost_census(const uint32_t* cl, const uint32_t* cr, uint8_t* cost, int W, int H,
-                              int D, int Dp, int m, hipStream_t s);
+hipError_t launch_cost_census(const uint8_t* left, const uint8_t* right, uint8_t* cost, int W,
+                              int H, int D, int Dp, int m, hipStream_t s);
 hipError_t launch_cost_sad(const uint8_t* left, const uint8_t* right, uint8_t* cost, int W, int H,
                            int D, int Dp, int m, int blockSize, int preFilterCap, hipStream_t s);
 hipError_t launch_paths(PathsArgs& a, hipStream_t s);

@@ -94,77 +94,79 @@ __device__ __forceinline__ float parabola(float s, uint32_t a0, uint32_t a1, uin
 }
 
 // ---------------------------------------------------------------------------------------
-// Census transform (src/sgbm_kernels.cu:217-245): both images in one launch (blockIdx.z).
-// 64x4 pixel tiles staged with a 2-pixel halo in LDS.
-// ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void census5x5_kernel(const uint8_t* __restrict__ img0,
-                                                        const uint8_t* __restrict__ img1,
-                                                        uint32_t* __restrict__ out0,
-                                                        uint32_t* __restrict__ out1, int W,
-                                                        int H) {
-    constexpr int TW = 64, TH = 4;
-    __shared__ uint8_t tile[TH + 4][TW + 4];
-    const uint8_t* img = blockIdx.z ? img1 : img0;
-    uint32_t* out = blockIdx.z ? out1 : out0;
-    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
-    for (int i = threadIdx.x; i < (TH + 4) * (TW + 4); i += 256) {
-        const int ty = i / (TW + 4), tx = i % (TW + 4);
-        const int gx = x0 + tx - 2, gy = y0 + ty - 2;
-        tile[ty][tx] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? img[(size_t)gy * W + gx] : 0;
-    }
-    __syncthreads();
-    const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= W || y >= H) return;
-    uint32_t code = 0;
-    if (x >= 2 && x < W - 2 && y >= 2 && y < H - 2) {
-        const uint8_t c = tile[ty + 2][tx + 2];
-        int k = 0;
-#pragma unroll
-        for (int dy = 0; dy < 5; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 5; ++dx) {
-                if (dy == 2 && dx == 2) continue;
-                code |= (tile[ty + dy][tx + dx] >= c ? 1u : 0u) << k;
-                ++k;
-            }
-    }
-    out[(size_t)y * W + x] = code;
-}
-
-// ---------------------------------------------------------------------------------------
-// Census cost (src/sgbm_kernels.cu:250-286): one block per (row, 64-pixel segment); the
-// segment's left codes and the right codes it can reach are staged in LDS, and each thread
-// writes 16 consecutive disparities of one pixel as one 16-byte store.
+// Census cost (src/sgbm_kernels.cu:217-286), census transform fused in. One block per
+// (row, 64-pixel segment): the five image rows around the row are staged in LDS, the 24-bit
+// census codes of the segment and of every right pixel it can reach are computed into LDS
+// (bit k over the 5x5 neighbours in row-major order, centre skipped, set when the neighbour
+// is >= the centre; code 0 within 2 px of the border), then each thread writes 16
+// consecutive disparities of one pixel as one 16-byte store.
 // ---------------------------------------------------------------------------------------
 constexpr int kCostTile = 64;
 
-__global__ __launch_bounds__(256) void cost_census_kernel(const uint32_t* __restrict__ cl,
-                                                          const uint32_t* __restrict__ cr,
+__device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, int c) {
+    // rows: 5 staged image rows, column c is the centre's column in the staging buffer.
+    const uint8_t ctr = rows[2 * pitch + c];
+    uint32_t code = 0;
+    int k = 0;
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            if (dy == 2 && dx == 0) continue;
+            code |= (rows[dy * pitch + c + dx] >= ctr ? 1u : 0u) << k;
+            ++k;
+        }
+    return code;
+}
+
+__global__ __launch_bounds__(256) void cost_census_kernel(const uint8_t* __restrict__ left,
+                                                          const uint8_t* __restrict__ right,
                                                           uint8_t* __restrict__ cost, int W,
                                                           int H, int D, int Dp, int m) {
     extern __shared__ uint32_t shc[];
-    uint32_t* sl = shc;
-    uint32_t* sr = shc + kCostTile;
     const int y = blockIdx.y, x0 = blockIdx.x * kCostTile;
-    const int rs = x0 - m - (Dp - 1);
-    for (int i = threadIdx.x; i < kCostTile; i += 256) {
-        const int x = x0 + i;
-        sl[i] = x < W ? cl[(size_t)y * W + x] : 0u;
-    }
-    for (int i = threadIdx.x; i < kCostTile + Dp - 1; i += 256) {
-        const int rx = rs + i;
-        sr[i] = (rx >= 0 && rx < W) ? cr[(size_t)y * W + rx] : 0u;
-    }
-    __syncthreads();
+    const int rs = x0 - m - (Dp - 1);           // first right pixel the segment can reach
+    const int nr = kCostTile + Dp - 1;          // right pixels the segment can reach
+    const int lpitch = kCostTile + 4, rpitch = nr + 4;
+    uint32_t* sl = shc;                         // kCostTile codes
+    uint32_t* sr = shc + kCostTile;             // nr codes
+    uint8_t* il = reinterpret_cast<uint8_t*>(shc + kCostTile + nr);  // 5 x lpitch bytes
+    uint8_t* ir = il + 5 * lpitch;                                     // 5 x rpitch bytes
     const bool rowOk = y >= 2 && y < H - 2;
+    if (rowOk) {
+        for (int i = threadIdx.x; i < 5 * lpitch; i += 256) {
+            const int r = i / lpitch, c = x0 - 2 + (i - r * lpitch);
+            il[i] = (c >= 0 && c < W) ? left[(size_t)(y - 2 + r) * W + c] : uint8_t(0);
+        }
+        for (int i = threadIdx.x; i < 5 * rpitch; i += 256) {
+            const int r = i / rpitch, c = rs - 2 + (i - r * rpitch);
+            ir[i] = (c >= 0 && c < W) ? right[(size_t)(y - 2 + r) * W + c] : uint8_t(0);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < kCostTile; i += 256) {
+            const int x = x0 + i;
+            sl[i] = (x >= 2 && x < W - 2) ? census_at(il, lpitch, i + 2) : 0u;
+        }
+        for (int i = threadIdx.x; i < nr; i += 256) {
+            const int rx = rs + i;
+            sr[i] = (rx >= 2 && rx < W - 2) ? census_at(ir, rpitch, i + 2) : 0u;
+        }
+        __syncthreads();
+    }
     const int chunks = Dp / 16;
-    for (int item = threadIdx.x; item < kCostTile * chunks; item += 256) {
-        const int i = item / chunks, c = item - i * chunks;
+    // item = pixel * chunks + chunk, advanced by 256 items without a division per item.
+    int i = threadIdx.x / chunks, c = threadIdx.x - i * chunks;
+    const int di = 256 / chunks, dc = 256 - di * chunks;
+    for (; i < kCostTile; i += di, c += dc) {
+        if (c >= chunks) {
+            c -= chunks;
+            ++i;
+            if (i >= kCostTile) break;
+        }
         const int x = x0 + i;
-        if (x >= W) continue;
+        if (x >= W) break;
         const bool leftOk = rowOk && x >= 2 && x < W - 2;
-        const uint32_t a = sl[i];
+        const uint32_t a = leftOk ? sl[i] : 0u;
         const int d0 = c * 16;
         uint32_t w[4];
 #pragma unroll
@@ -834,45 +836,53 @@ __global__ __launch_bounds__(256) void speckle_kernel(const int16_t* __restrict_
 }
 
 // ---------------------------------------------------------------------------------------
-// 3x3 median of positive values (src/sgbm_kernels.cu:892-935): 16x16 tiles with a 1-pixel halo
-// in LDS; out-of-image taps are staged as 0, which the positive-only rule excludes.
+// 3x3 median of positive values (src/sgbm_kernels.cu:892-935): 64x4 tiles with a 1-pixel halo
+// in LDS; out-of-image taps are staged as 0, which the positive-only rule excludes. The nine
+// taps (non-positive ones replaced by INT_MAX so they sort last) go through a fixed
+// 25-comparator sorting network; the upper median is element n/2 of the sorted positives.
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void cswap(int& a, int& b) {
+    const int lo = min(a, b);
+    b = max(a, b);
+    a = lo;
+}
+
 __global__ __launch_bounds__(256) void median3x3_kernel(const int16_t* __restrict__ in,
                                                         int16_t* __restrict__ out, int W, int H) {
-    __shared__ int16_t tile[18][18];
-    const int x0 = blockIdx.x * 16, y0 = blockIdx.y * 16;
-    for (int i = threadIdx.x; i < 18 * 18; i += 256) {
-        const int ty = i / 18, tx = i % 18;
+    constexpr int TW = 64, TH = 4;
+    __shared__ int16_t tile[TH + 2][TW + 2];
+    const int x0 = blockIdx.x * TW, y0 = blockIdx.y * TH;
+    for (int i = threadIdx.x; i < (TH + 2) * (TW + 2); i += 256) {
+        const int ty = i / (TW + 2), tx = i % (TW + 2);
         const int gx = x0 + tx - 1, gy = y0 + ty - 1;
         tile[ty][tx] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? in[(size_t)gy * W + gx] : int16_t(0);
     }
     __syncthreads();
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int tx = threadIdx.x % TW, ty = threadIdx.x / TW;
     const int x = x0 + tx, y = y0 + ty;
     if (x >= W || y >= H) return;
     int v[9];
     int n = 0;
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy)
+    for (int k = 0; k < 9; ++k) {
+        const int t = tile[ty + k / 3][tx + k % 3];
+        n += t > 0 ? 1 : 0;
+        v[k] = t > 0 ? t : 0x7FFFFFFF;
+    }
+    cswap(v[0], v[1]); cswap(v[3], v[4]); cswap(v[6], v[7]);
+    cswap(v[1], v[2]); cswap(v[4], v[5]); cswap(v[7], v[8]);
+    cswap(v[0], v[1]); cswap(v[3], v[4]); cswap(v[6], v[7]);
+    cswap(v[0], v[3]); cswap(v[3], v[6]); cswap(v[0], v[3]);
+    cswap(v[1], v[4]); cswap(v[4], v[7]); cswap(v[1], v[4]);
+    cswap(v[2], v[5]); cswap(v[5], v[8]); cswap(v[2], v[5]);
+    cswap(v[1], v[3]); cswap(v[5], v[7]); cswap(v[2], v[6]);
+    cswap(v[4], v[6]); cswap(v[2], v[4]); cswap(v[2], v[3]);
+    cswap(v[5], v[6]);
+    const int k = n >> 1;
+    int r = v[0];
 #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) {
-            const int t = tile[ty + dy][tx + dx];
-            if (t > 0) v[n++] = t;
-        }
-    if (n == 0) {
-        out[(size_t)y * W + x] = tile[ty + 1][tx + 1];
-        return;
-    }
-    for (int i = 1; i < n; ++i) {
-        const int key = v[i];
-        int j = i - 1;
-        while (j >= 0 && v[j] > key) {
-            v[j + 1] = v[j];
-            --j;
-        }
-        v[j + 1] = key;
-    }
-    out[(size_t)y * W + x] = static_cast<int16_t>(v[n / 2]);
+    for (int j = 1; j < 9; ++j) r = k == j ? v[j] : r;
+    out[(size_t)y * W + x] = n == 0 ? tile[ty + 1][tx + 1] : static_cast<int16_t>(r);
 }
 
 // Sum of all path costs mod 2^16 into a padded volume (parity dumps only).
@@ -889,18 +899,13 @@ __global__ __launch_bounds__(256) void path_sum_kernel(const uint16_t* __restric
 // ---------------------------------------------------------------------------------------
 // Launchers.
 // ---------------------------------------------------------------------------------------
-hipError_t launch_census(const uint8_t* left, const uint8_t* right, uint32_t* cl, uint32_t* cr,
-                         int W, int H, hipStream_t s) {
-    dim3 grid((W + 63) / 64, (H + 3) / 4, 2);
-    hipLaunchKernelGGL(census5x5_kernel, grid, dim3(256), 0, s, left, right, cl, cr, W, H);
-    return hipGetLastError();
-}
-
-hipError_t launch_cost_census(const uint32_t* cl, const uint32_t* cr, uint8_t* cost, int W, int H,
-                              int D, int Dp, int m, hipStream_t s) {
+hipError_t launch_cost_census(const uint8_t* left, const uint8_t* right, uint8_t* cost, int W,
+                              int H, int D, int Dp, int m, hipStream_t s) {
     dim3 grid((W + kCostTile - 1) / kCostTile, H);
-    const size_t lds = (size_t)(2 * kCostTile + Dp) * sizeof(uint32_t);
-    hipLaunchKernelGGL(cost_census_kernel, grid, dim3(256), lds, s, cl, cr, cost, W, H, D, Dp, m);
+    const int nr = kCostTile + Dp - 1;
+    const size_t lds = (size_t)(kCostTile + nr) * 4 + 5 * (kCostTile + 4) + 5 * (nr + 4);
+    hipLaunchKernelGGL(cost_census_kernel, grid, dim3(256), lds, s, left, right, cost, W, H, D,
+                       Dp, m);
     return hipGetLastError();
 }
 
@@ -933,7 +938,7 @@ hipError_t launch_speckle(const int16_t* in, int16_t* out, int W, int H, int min
 }
 
 hipError_t launch_median(const int16_t* in, int16_t* out, int W, int H, hipStream_t s) {
-    dim3 grid((W + 15) / 16, (H + 15) / 16);
+    dim3 grid((W + 63) / 64, (H + 3) / 4);
     hipLaunchKernelGGL(median3x3_kernel, grid, dim3(256), 0, s, in, out, W, H);
     return hipGetLastError();
 }

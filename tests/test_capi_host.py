@@ -107,7 +107,7 @@ def test_footprint():
                         matchingCostMode=ls.MatchingCostMode.Census).to_c()
     assert lib.lumen_sgbm_footprint(ctypes.byref(p), 1920, 1080, ctypes.byref(out)) == 0
     n = 1920 * 1080
-    assert out.value == n * 128 * (1 + 2 * 8) + n * 2 + n * 12 + n * 8
+    assert out.value == n * 128 * (1 + 2 * 8) + n * 2 + n * 12
     p = ls.StereoParams(maxDisparity=290, numDirections=4).to_c()   # padded to 320
     assert lib.lumen_sgbm_footprint(ctypes.byref(p), 100, 10, ctypes.byref(out)) == 0
     assert out.value == 1000 * 320 * 9 + 1000 * 14
