@@ -80,6 +80,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a plain copy of "
+                         "known size, to calibrate rocprof FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the matcher "
@@ -117,6 +120,15 @@ def main() -> int:
     agg_bytes = de * 3 * npaths
     achieved = agg_bytes / (agg_ms * 1e-3) / 1e9
     frame_bytes = de * (1 + 5 * npaths)
+
+    if args.copy_calibration_mb > 0:
+        n = args.copy_calibration_mb << 20
+        src = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     gpu_out = d_out.cpu().numpy()
     line = {

@@ -10,8 +10,9 @@ namespace lumen {
 constexpr int kScanLanes = 16;
 constexpr int kMaxPaths = 8;
 constexpr int kMaxDisparityRange = 512;
-// The fused sum/WTA kernel keeps one row of right-WTA state in LDS: 12 bytes per pixel.
-constexpr int kMaxWidth = 5376;
+// The fused sum/WTA kernel keeps one row of right-WTA state in LDS: up to 14 bytes per pixel
+// (8-byte keys with one padding entry per 4, plus two int16 maps) within 64 KiB.
+constexpr int kMaxWidth = 4600;
 
 // Elements of the aggregation kernel's trash buffer (64 lanes x 32 entries).
 constexpr int kTrashElems = 64 * 32;

@@ -95,13 +95,14 @@ __device__ __forceinline__ float parabola(float s, uint32_t a0, uint32_t a1, uin
 
 // ---------------------------------------------------------------------------------------
 // Census cost (src/sgbm_kernels.cu:217-286), census transform fused in. One block per
-// (row, 64-pixel segment): the five image rows around the row are staged in LDS, the 24-bit
+// (row, 256-pixel segment): the five image rows around the row are staged in LDS, the 24-bit
 // census codes of the segment and of every right pixel it can reach are computed into LDS
 // (bit k over the 5x5 neighbours in row-major order, centre skipped, set when the neighbour
 // is >= the centre; code 0 within 2 px of the border), then each thread writes 16
 // consecutive disparities of one pixel as one 16-byte store.
 // ---------------------------------------------------------------------------------------
-constexpr int kCostTile = 64;
+constexpr int kCensusTile = 256;  // pixels per block of the census cost kernel
+constexpr int kCostTile = 64;     // pixels per block of the SAD cost kernel
 
 __device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, int c) {
     // rows: 5 staged image rows, column c is the centre's column in the staging buffer.
@@ -123,27 +124,33 @@ __global__ __launch_bounds__(256) void cost_census_kernel(const uint8_t* __restr
                                                           const uint8_t* __restrict__ right,
                                                           uint8_t* __restrict__ cost, int W,
                                                           int H, int D, int Dp, int m) {
+    constexpr int T = kCensusTile;
     extern __shared__ uint32_t shc[];
-    const int y = blockIdx.y, x0 = blockIdx.x * kCostTile;
+    const int y = blockIdx.y, x0 = blockIdx.x * T;
     const int rs = x0 - m - (Dp - 1);           // first right pixel the segment can reach
-    const int nr = kCostTile + Dp - 1;          // right pixels the segment can reach
-    const int lpitch = kCostTile + 4, rpitch = nr + 4;
-    uint32_t* sl = shc;                         // kCostTile codes
-    uint32_t* sr = shc + kCostTile;             // nr codes
-    uint8_t* il = reinterpret_cast<uint8_t*>(shc + kCostTile + nr);  // 5 x lpitch bytes
-    uint8_t* ir = il + 5 * lpitch;                                     // 5 x rpitch bytes
+    const int nr = T + Dp - 1;                  // right pixels the segment can reach
+    const int lpitch = T + 4, rpitch = nr + 4;
+    uint32_t* sl = shc;                         // T codes
+    uint32_t* sr = shc + T;                     // nr codes
+    uint8_t* il = reinterpret_cast<uint8_t*>(shc + T + nr);  // 5 x lpitch bytes
+    uint8_t* ir = il + 5 * lpitch;                             // 5 x rpitch bytes
     const bool rowOk = y >= 2 && y < H - 2;
     if (rowOk) {
-        for (int i = threadIdx.x; i < 5 * lpitch; i += 256) {
-            const int r = i / lpitch, c = x0 - 2 + (i - r * lpitch);
-            il[i] = (c >= 0 && c < W) ? left[(size_t)(y - 2 + r) * W + c] : uint8_t(0);
-        }
-        for (int i = threadIdx.x; i < 5 * rpitch; i += 256) {
-            const int r = i / rpitch, c = rs - 2 + (i - r * rpitch);
-            ir[i] = (c >= 0 && c < W) ? right[(size_t)(y - 2 + r) * W + c] : uint8_t(0);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const uint8_t* lrow = left + (size_t)(y - 2 + r) * W;
+            const uint8_t* rrow = right + (size_t)(y - 2 + r) * W;
+            for (int i = threadIdx.x; i < lpitch; i += 256) {
+                const int c = x0 - 2 + i;
+                il[r * lpitch + i] = (c >= 0 && c < W) ? lrow[c] : uint8_t(0);
+            }
+            for (int i = threadIdx.x; i < rpitch; i += 256) {
+                const int c = rs - 2 + i;
+                ir[r * rpitch + i] = (c >= 0 && c < W) ? rrow[c] : uint8_t(0);
+            }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < kCostTile; i += 256) {
+        for (int i = threadIdx.x; i < T; i += 256) {
             const int x = x0 + i;
             sl[i] = (x >= 2 && x < W - 2) ? census_at(il, lpitch, i + 2) : 0u;
         }
@@ -157,33 +164,51 @@ __global__ __launch_bounds__(256) void cost_census_kernel(const uint8_t* __restr
     // item = pixel * chunks + chunk, advanced by 256 items without a division per item.
     int i = threadIdx.x / chunks, c = threadIdx.x - i * chunks;
     const int di = 256 / chunks, dc = 256 - di * chunks;
-    for (; i < kCostTile; i += di, c += dc) {
+    for (; i < T; i += di, c += dc) {
         if (c >= chunks) {
             c -= chunks;
             ++i;
-            if (i >= kCostTile) break;
+            if (i >= T) break;
         }
         const int x = x0 + i;
         if (x >= W) break;
         const bool leftOk = rowOk && x >= 2 && x < W - 2;
         const uint32_t a = leftOk ? sl[i] : 0u;
         const int d0 = c * 16;
+        const uint32_t* rc = sr + i + Dp - 1 - d0;   // rc[-j] is the right code for d0 + j
         uint32_t w[4];
+        if (leftOk && d0 + 15 < D && x - m - d0 - 15 >= 2 && x - m - d0 < W - 2) {
+            // Interior: all 16 disparities valid. (h*255+12)/24 == (h*2718+168) >> 8 for
+            // h in [0,24]; two results per packed 16-bit multiply-add, byte 1 of each half
+            // gathered with one byte permute per four results.
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t word = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int d = d0 + q * 4 + b;
-                const int rx = x - m - d;
-                uint32_t v = 255;
-                if (leftOk && d < D && rx >= 2 && rx < W - 2) {
-                    const uint32_t h = __builtin_popcount(a ^ sr[i + Dp - 1 - d]);
-                    v = (h * 255u + 12u) / 24u;
-                }
-                word |= v << (8 * b);
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t h0 = __builtin_popcount(a ^ rc[-(4 * q + 0)]);
+                const uint32_t h1 = __builtin_popcount(a ^ rc[-(4 * q + 1)]);
+                const uint32_t h2 = __builtin_popcount(a ^ rc[-(4 * q + 2)]);
+                const uint32_t h3 = __builtin_popcount(a ^ rc[-(4 * q + 3)]);
+                const u16x2 k = splat(2718u), b = splat(168u);
+                const u16x2 v01 = as16(h0 | (h1 << 16)) * k + b;
+                const u16x2 v23 = as16(h2 | (h3 << 16)) * k + b;
+                w[q] = __builtin_amdgcn_perm(bits(v23), bits(v01), 0x07050301u);
             }
-            w[q] = word;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int d = d0 + q * 4 + b;
+                    const int rx = x - m - d;
+                    uint32_t v = 255;
+                    if (leftOk && d < D && rx >= 2 && rx < W - 2) {
+                        const uint32_t h = __builtin_popcount(a ^ rc[-(4 * q + b)]);
+                        v = (h * 255u + 12u) / 24u;
+                    }
+                    word |= v << (8 * b);
+                }
+                w[q] = word;
+            }
         }
         *reinterpret_cast<u32x4*>(cost + ((size_t)y * W + x) * Dp + d0) =
             u32x4{w[0], w[1], w[2], w[3]};
@@ -460,6 +485,18 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PathsArgs a) {
     const int s = (wave - a.waveStart[path]) * 4 + g;
     int p0, step, len;
     scan_geom(path, s, a.W, a.H, p0, step, len);
+    // The launch ends when its longest scanlines end, so waves with longer scanlines get
+    // higher issue priority within their SIMD: horizontal > vertical > diagonal when W >= H.
+    {
+        const int ph = a.W >= a.H ? (path < 2 ? 3 : (path < 4 ? 2 : 1))
+                                  : (path < 2 ? 2 : (path < 4 ? 3 : 1));
+        if (ph == 3)
+            __builtin_amdgcn_s_setprio(3);
+        else if (ph == 2)
+            __builtin_amdgcn_s_setprio(2);
+        else
+            __builtin_amdgcn_s_setprio(1);
+    }
     int maxLen = max(len, __shfl_xor(len, 16));
     maxLen = max(maxLen, __shfl_xor(maxLen, 32));
     maxLen = __builtin_amdgcn_readfirstlane(maxLen);
@@ -640,14 +677,20 @@ __device__ __forceinline__ uint32_t pick32(const uint32_t (&v)[K / 2], int slot)
     return r;
 }
 
+constexpr int kWtaThreads = 256;
+
 template <int K, bool PAD>
-__global__ __launch_bounds__(256) void wta_row_kernel(WtaArgs a) {
+__global__ __launch_bounds__(kWtaThreads) void wta_row_kernel(WtaArgs a) {
+    // rbest[slot(xr)]: running right-WTA key of right pixel xr. One padding entry every K
+    // entries spreads the 16 lanes of a row (whose candidates are K right pixels apart) over
+    // distinct LDS banks.
     extern __shared__ unsigned long long rbest[];
     const int W = a.W, D = a.D, Dp = a.Dp, m = a.m;
-    int16_t* sdl = reinterpret_cast<int16_t*>(rbest + W);
+    const int nslots = W + W / K + 1;
+    int16_t* sdl = reinterpret_cast<int16_t*>(rbest + nslots);
     int16_t* sdr = sdl + W;
     const int y = blockIdx.x;
-    for (int i = threadIdx.x; i < W; i += 256) rbest[i] = ~0ull;
+    for (int i = threadIdx.x; i < nslots; i += kWtaThreads) rbest[i] = ~0ull;
     __syncthreads();
     const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
     uint32_t padm[K / 2];
@@ -657,7 +700,7 @@ __global__ __launch_bounds__(256) void wta_row_kernel(WtaArgs a) {
         padm[j] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
     }
     const size_t rowPix = (size_t)y * W;
-    for (int x = g; x < W; x += 16) {
+    for (int x = g; x < W; x += kWtaThreads / 16) {
         const uint16_t* src = a.L + (rowPix + x) * Dp + l * K;
         u16x2 S[K / 2];
 #pragma unroll
@@ -747,14 +790,14 @@ __global__ __launch_bounds__(256) void wta_row_kernel(WtaArgs a) {
                     const unsigned long long key = (static_cast<unsigned long long>(v) << 48) |
                                                    (static_cast<unsigned long long>(d) << 32) |
                                                    (lo << 16) | hi;
-                    atomicMin(&rbest[xr], key);
+                    atomicMin(&rbest[xr + xr / K], key);
                 }
             }
         }
     }
     __syncthreads();
-    for (int xr = threadIdx.x; xr < W; xr += 256) {
-        const unsigned long long key = rbest[xr];
+    for (int xr = threadIdx.x; xr < W; xr += kWtaThreads) {
+        const unsigned long long key = rbest[xr + xr / K];
         float sub = 0.0f;
         if (key != ~0ull) {
             const int bd = static_cast<int>((key >> 32) & 0xFFFFu);
@@ -768,7 +811,7 @@ __global__ __launch_bounds__(256) void wta_row_kernel(WtaArgs a) {
         sdr[xr] = dr;
     }
     __syncthreads();
-    for (int x = threadIdx.x; x < W; x += 256) {
+    for (int x = threadIdx.x; x < W; x += kWtaThreads) {
         int16_t dl = sdl[x];
         if (a.doLR && dl > 0) {
             const int xr = x - dl / 16;
@@ -785,8 +828,8 @@ __global__ __launch_bounds__(256) void wta_row_kernel(WtaArgs a) {
 
 template <int K, bool PAD>
 static hipError_t launch_wta_k(const WtaArgs& a, hipStream_t s) {
-    const size_t lds = (size_t)a.W * 12;
-    hipLaunchKernelGGL((wta_row_kernel<K, PAD>), dim3(a.H), dim3(256), lds, s, a);
+    const size_t lds = (size_t)(a.W + a.W / K + 1) * 8 + (size_t)a.W * 4;
+    hipLaunchKernelGGL((wta_row_kernel<K, PAD>), dim3(a.H), dim3(kWtaThreads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -901,9 +944,9 @@ __global__ __launch_bounds__(256) void path_sum_kernel(const uint16_t* __restric
 // ---------------------------------------------------------------------------------------
 hipError_t launch_cost_census(const uint8_t* left, const uint8_t* right, uint8_t* cost, int W,
                               int H, int D, int Dp, int m, hipStream_t s) {
-    dim3 grid((W + kCostTile - 1) / kCostTile, H);
-    const int nr = kCostTile + Dp - 1;
-    const size_t lds = (size_t)(kCostTile + nr) * 4 + 5 * (kCostTile + 4) + 5 * (nr + 4);
+    dim3 grid((W + kCensusTile - 1) / kCensusTile, H);
+    const int nr = kCensusTile + Dp - 1;
+    const size_t lds = (size_t)(kCensusTile + nr) * 4 + 5 * (kCensusTile + 4) + 5 * (nr + 4);
     hipLaunchKernelGGL(cost_census_kernel, grid, dim3(256), lds, s, left, right, cost, W, H, D,
                        Dp, m);
     return hipGetLastError();
