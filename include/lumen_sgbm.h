@@ -136,6 +136,39 @@ int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int heigh
 int lumen_synth_pair(int width, int height, float dmin, float dmax, uint64_t seed,
                      float lowtex_frac, uint8_t* left, uint8_t* right, float* gt);
 
+/* ------------------------------------------------------------------------------------------
+ * Batched form (no reference counterpart). Many pairs of different size, disparity range, cost
+ * mode and path count are processed stage by stage in persistent kernels over work lists, the
+ * whole pipeline replayed as one captured hipGraph. Every pair's disparity is bit-identical to
+ * lumen_sgbm_compute of the same pair with the same parameters.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct lumen_pair_desc {
+    lumen_stereo_params params;
+    int32_t width;
+    int32_t height;
+    const uint8_t* left;   /* packed grayscale, host or device memory (see device_io) */
+    const uint8_t* right;
+    int16_t* out;          /* packed int16 disparity, host or device memory */
+} lumen_pair_desc;
+
+typedef struct lumen_batch lumen_batch;
+
+/* A batch executor with a device arena of arena_bytes (0 = half of the free device memory) for
+ * the pairs' working sets; a batch larger than the arena runs as several sub-batches. */
+int lumen_batch_create(size_t arena_bytes, lumen_batch** out);
+int lumen_batch_destroy(lumen_batch* b);
+int lumen_batch_arena_bytes(const lumen_batch* b, size_t* out);
+
+/* Runs npairs pairs. device_io = 0: images and outputs are host memory (staged through pinned
+ * buffers); device_io = 1: they are device memory. Blocking. Pairs are validated as
+ * lumen_sgbm_compute validates them. */
+int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs, int device_io);
+
+/* Times of the last run in ms: out[0] total, out[1] upload, out[2] compute, out[3] download. */
+int lumen_batch_get_timings(const lumen_batch* b, float out[4]);
+
+const char* lumen_batch_last_error(const lumen_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,7 @@ __all__ = [
     "library",
     "LIB_PATH",
     "STAGE",
+    "BatchSGBM",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "liblumen_sgbm.so"
@@ -149,6 +150,13 @@ class CParams(ctypes.Structure):
         "useSharedMemory", "numDirections", "matchingCostMode")]
 
 
+class CPairDesc(ctypes.Structure):
+    """lumen_pair_desc (include/lumen_sgbm.h)."""
+
+    _fields_ = [("params", CParams), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("left", ctypes.c_void_p), ("right", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
 _lib = None
 
 # Every symbol include/lumen_sgbm.h declares, with its ctypes signature.
@@ -175,6 +183,12 @@ _SIGNATURES = {
     "lumen_synth_pair": (_c_int, [_c_int, _c_int, ctypes.c_float, ctypes.c_float,
                                   ctypes.c_uint64, ctypes.c_float, _c_void_p, _c_void_p,
                                   _c_void_p]),
+    "lumen_batch_create": (_c_int, [_c_size_t, ctypes.POINTER(_c_void_p)]),
+    "lumen_batch_destroy": (_c_int, [_c_void_p]),
+    "lumen_batch_arena_bytes": (_c_int, [_c_void_p, ctypes.POINTER(_c_size_t)]),
+    "lumen_batch_run": (_c_int, [_c_void_p, ctypes.POINTER(CPairDesc), _c_int, _c_int]),
+    "lumen_batch_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "lumen_batch_last_error": (ctypes.c_char_p, [_c_void_p]),
 }
 
 
@@ -337,3 +351,64 @@ class StereoSGBM:
 
     def getPostProcessTimeMs(self) -> float:
         return float(self._times[3])
+
+
+class BatchSGBM:
+    """Batched form: many pairs per stage in persistent kernels replayed as one hipGraph
+    (lumen_batch_* in include/lumen_sgbm.h). Outputs are bit-identical to StereoSGBM."""
+
+    def __init__(self, arena_bytes: int = 0):
+        self._lib = library()
+        self._h = ctypes.c_void_p()
+        rc = self._lib.lumen_batch_create(int(arena_bytes), ctypes.byref(self._h))
+        if rc != LUMEN_OK:
+            raise GpuException(self._lib.lumen_batch_last_error(None).decode(), rc)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.lumen_batch_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def arena_bytes(self) -> int:
+        v = ctypes.c_size_t()
+        self._lib.lumen_batch_arena_bytes(self._h, ctypes.byref(v))
+        return v.value
+
+    def _raise(self, rc):
+        msg = self._lib.lumen_batch_last_error(self._h).decode()
+        if rc == LUMEN_EINVAL:
+            raise StereoException(msg)
+        raise GpuException(msg, rc)
+
+    def run(self, pairs):
+        """pairs: iterable of (StereoParams, left uint8 HxW, right uint8 HxW). Returns the int16
+        disparity maps, in order."""
+        pairs = list(pairs)
+        descs = (CPairDesc * max(len(pairs), 1))()
+        keep, outs = [], []
+        for i, (p, l, r) in enumerate(pairs):
+            l = np.ascontiguousarray(l, np.uint8)
+            r = np.ascontiguousarray(r, np.uint8)
+            if l.shape != r.shape or l.ndim != 2:
+                raise StereoException("Left and right images must have same size")
+            o = np.empty(l.shape, np.int16)
+            keep += [l, r]
+            outs.append(o)
+            descs[i] = CPairDesc(p.to_c(), l.shape[1], l.shape[0], _ptr(l), _ptr(r), _ptr(o))
+        rc = self._lib.lumen_batch_run(self._h, descs, len(pairs), 0)
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        return outs
+
+    def run_device(self, descs, count: int):
+        """descs: a ctypes array of CPairDesc whose pointers are device memory."""
+        rc = self._lib.lumen_batch_run(self._h, descs, count, 1)
+        if rc != LUMEN_OK:
+            self._raise(rc)
+
+    def timings(self) -> np.ndarray:
+        """[total, upload, compute, download] of the last run in ms."""
+        t = np.zeros(4, np.float32)
+        self._lib.lumen_batch_get_timings(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+        return t

@@ -19,7 +19,7 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_capi.hip"]
+HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_capi.hip", "sgbm_batch.hip"]
 CXX_SOURCES = ["synth_scene.cpp"]
 
 HIP_FLAGS = [

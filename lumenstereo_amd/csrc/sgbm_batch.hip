@@ -1,0 +1,732 @@
+// sgbm_batch.hip - batched form: many stereo pairs of different size, disparity range, cost
+// mode and path count per stage, in persistent kernels driven by work lists, replayed as one
+// hipGraph.
+//
+// Each stage is one persistent kernel whose blocks (or waves, for the aggregation) take units
+// of work from an atomic counter until the list is exhausted:
+//   cost        (pair, row, segment)       block-wide  census or SAD by the pair's mode
+//   paths       (pair, path, 4 scanlines)  wave-wide   list sorted by scanline length, longest
+//                                                      first, so the stage's tail stays short
+//   sum/WTA/LR  (pair, row)                block-wide
+//   speckle     (pair, 16x16 tile)         block-wide  only pairs with the filter enabled
+//   median      (pair, 64x4 tile)          block-wide
+// The units are the same device bodies the per-pair kernels run (sgbm_device.h), so every
+// pair's disparity is bit-identical to the per-pair path. Grids are sized to what fits on the
+// device at once, so a captured graph does not depend on the batch it processes; the work
+// lists and the pair table are rewritten between replays.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lumen_sgbm.h"
+#include "sgbm_device.h"
+#include "sgbm_host.h"
+
+using namespace lumen;
+
+namespace {
+
+enum Counter { kCtrCost = 0, kCtrPaths, kCtrPathsBig, kCtrWta, kCtrWtaBig, kCtrSpeckle, kCtrMedian, kNumCounters };
+
+// Device-resident sizes of the current batch (so a captured graph serves any batch).
+enum HeaderField { kHdrPairs = 0, kHdrWta0, kHdrWta1, kHdrSpeckle, kHdrItems, kHdrItemsBig, kHdrFields };
+
+// Prefix tables, each with room for `cap` pairs: costStart, wta0Start, wta1Start, spkStart,
+// medStart ([cap+1] each), then wta0Pairs, wta1Pairs, spkPairs ([cap] each).
+struct Tables {
+    const int* hdr;
+    const int* costStart;
+    const int* wta0Start;
+    const int* wta1Start;
+    const int* spkStart;
+    const int* medStart;
+    const int* wta0Pairs;
+    const int* wta1Pairs;
+    const int* spkPairs;
+};
+
+struct PathItem {
+    uint16_t pair;
+    uint8_t path;
+    uint8_t K;
+    int32_t s0;
+};
+
+// Prefix sums of per-pair unit counts: unit u belongs to the pair p with start[p] <= u < start[p+1].
+__device__ __forceinline__ int find_pair(const int* __restrict__ start, int npairs, int u) {
+    int lo = 0, hi = npairs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= u)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// One thread of the block takes the next unit; everybody gets it.
+__device__ __forceinline__ int block_take(int* ctr, int* s_item) {
+    if (threadIdx.x == 0) *s_item = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int it = *s_item;
+    __syncthreads();
+    return it;
+}
+
+__global__ void reset_counters_kernel(int* ctr) {
+    if (threadIdx.x < kNumCounters) ctr[threadIdx.x] = 0;
+}
+
+// ----------------------------------------------------------------------------------- cost
+// Units: census pairs contribute ceil(W/256)*H segments, SAD pairs ceil(W/64)*H.
+__global__ __launch_bounds__(256) void batch_cost_kernel(const PairJob* __restrict__ jobs,
+                                                         Tables t, int* ctr) {
+    extern __shared__ uint32_t lds[];
+    __shared__ int s_item;
+    const int npairs = t.hdr[kHdrPairs];
+    const int* start = t.costStart;
+    const int total = start[npairs];
+    for (;;) {
+        const int u = block_take(&ctr[kCtrCost], &s_item);
+        if (u >= total) break;
+        const int p = find_pair(start, npairs, u);
+        const PairJob j = jobs[p];
+        const int local = u - start[p];
+        if (j.mode == 1) {
+            const int segs = (j.W + kCensusTile - 1) / kCensusTile;
+            const int y = local / segs;
+            census_cost_segment(j, y, (local - y * segs) * kCensusTile, lds);
+        } else {
+            const int segs = (j.W + kSadTile - 1) / kSadTile;
+            const int y = local / segs;
+            sad_cost_segment(j, y, (local - y * segs) * kSadTile, reinterpret_cast<uint8_t*>(lds));
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------------- paths
+// KSET 0 handles K in {4, 8, 12}, KSET 1 handles K in {16, ..., 32}: one instantiation's
+// register count is the largest of its cases, so the big slices get their own kernel.
+template <int KSET>
+__global__ __launch_bounds__(256) void batch_paths_kernel(const PairJob* __restrict__ jobs,
+                                                          const PathItem* __restrict__ items,
+                                                          Tables t, int* ctr, uint16_t* trash) {
+    const int lane = threadIdx.x & 63;
+    const int nitems = t.hdr[KSET == 0 ? kHdrItems : kHdrItemsBig];
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(&ctr[KSET == 0 ? kCtrPaths : kCtrPathsBig], 1);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= nitems) break;
+        const PathItem item = items[it];
+        const PairJob j = jobs[item.pair];
+        set_priority_for_length(scan_length(item.path, item.s0, j.W, j.H));
+        if constexpr (KSET == 0) {
+            switch (item.K) {
+                case 4: paths_wave<4, true>(j, item.path, item.s0, trash); break;
+                case 8: paths_wave<8, true>(j, item.path, item.s0, trash); break;
+                default: paths_wave<12, true>(j, item.path, item.s0, trash); break;
+            }
+        } else {
+            switch (item.K) {
+                case 16: paths_wave<16, true>(j, item.path, item.s0, trash); break;
+                case 20: paths_wave<20, true>(j, item.path, item.s0, trash); break;
+                case 24: paths_wave<24, true>(j, item.path, item.s0, trash); break;
+                case 28: paths_wave<28, true>(j, item.path, item.s0, trash); break;
+                default: paths_wave<32, true>(j, item.path, item.s0, trash); break;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------- WTA
+// Units: rows of the pairs in this K set (prefix sums over those pairs only).
+template <int KSET>
+__global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restrict__ jobs,
+                                                        Tables t, int* ctr) {
+    extern __shared__ unsigned long long rbest[];
+    __shared__ int s_item;
+    const int npairs = t.hdr[KSET == 0 ? kHdrWta0 : kHdrWta1];
+    const int* start = KSET == 0 ? t.wta0Start : t.wta1Start;
+    const int* pairIdx = KSET == 0 ? t.wta0Pairs : t.wta1Pairs;
+    const int total = start[npairs];
+    for (;;) {
+        const int u = block_take(&ctr[KSET == 0 ? kCtrWta : kCtrWtaBig], &s_item);
+        if (u >= total) break;
+        const int q = find_pair(start, npairs, u);
+        const PairJob j = jobs[pairIdx[q]];
+        const int y = u - start[q];
+        if constexpr (KSET == 0) {
+            switch (j.K) {
+                case 4: wta_row<4, true>(j, y, rbest); break;
+                case 8: wta_row<8, true>(j, y, rbest); break;
+                default: wta_row<12, true>(j, y, rbest); break;
+            }
+        } else {
+            switch (j.K) {
+                case 16: wta_row<16, true>(j, y, rbest); break;
+                case 20: wta_row<20, true>(j, y, rbest); break;
+                case 24: wta_row<24, true>(j, y, rbest); break;
+                case 28: wta_row<28, true>(j, y, rbest); break;
+                default: wta_row<32, true>(j, y, rbest); break;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------------------------- post
+__global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __restrict__ jobs,
+                                                            Tables t, int* ctr) {
+    __shared__ int s_item;
+    const int npairs = t.hdr[kHdrSpeckle];
+    const int* start = t.spkStart;
+    const int* pairIdx = t.spkPairs;
+    const int total = start[npairs];
+    for (;;) {
+        const int u = block_take(&ctr[kCtrSpeckle], &s_item);
+        if (u >= total) break;
+        const int q = find_pair(start, npairs, u);
+        const PairJob j = jobs[pairIdx[q]];
+        const int tiles = (j.W + 15) / 16;
+        const int local = u - start[q];
+        const int ty = local / tiles, tx = local - ty * tiles;
+        const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
+        if (x < j.W && y < j.H)
+            speckle_px(j.dispLR, j.spk, j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
+    }
+}
+
+__global__ __launch_bounds__(256) void batch_median_kernel(const PairJob* __restrict__ jobs,
+                                                           Tables t, int* ctr) {
+    __shared__ int16_t tile[kMedTH + 2][kMedTW + 2];
+    __shared__ int s_item;
+    const int npairs = t.hdr[kHdrPairs];
+    const int* start = t.medStart;
+    const int total = start[npairs];
+    for (;;) {
+        const int u = block_take(&ctr[kCtrMedian], &s_item);
+        if (u >= total) break;
+        const int p = find_pair(start, npairs, u);
+        const PairJob j = jobs[p];
+        const int tiles = (j.W + kMedTW - 1) / kMedTW;
+        const int local = u - start[p];
+        const int ty = local / tiles, tx = local - ty * tiles;
+        median_tile(j.doSpeckle ? j.spk : j.dispLR, j.out, j.W, j.H, tx * kMedTW, ty * kMedTH, tile);
+        __syncthreads();
+    }
+}
+
+thread_local std::string g_batch_error;
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+template <typename F>
+int occupancy_blocks(F kernel, size_t lds) {
+    int perCU = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, lds) != hipSuccess || perCU < 1)
+        perCU = 1;
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    return perCU * std::max(cus, 1);
+}
+
+}  // namespace
+
+struct lumen_batch {
+    std::string err;
+    bool ready = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    // Device arena for the pairs' working sets.
+    void* arena = nullptr;
+    size_t arenaBytes = 0;
+    // Device tables (grow on demand).
+    PairJob* jobs = nullptr;
+    size_t jobsCap = 0;
+    PathItem* items = nullptr;     // aggregation units of slices with K <= 12
+    size_t itemsCap = 0;
+    PathItem* itemsBig = nullptr;  // aggregation units of slices with K > 12
+    size_t itemsBigCap = 0;
+    int* tables = nullptr;         // header + prefix tables (see Tables)
+    size_t pairCap = 0;            // pairs the prefix tables have room for
+    int* ctr = nullptr;
+    uint16_t* trash = nullptr;
+    // Pinned host staging for host-memory batches.
+    uint8_t* hostIn = nullptr;
+    size_t hostInCap = 0;
+    int16_t* hostOut = nullptr;
+    size_t hostOutCap = 0;
+    // Captured pipeline, valid for the LDS sizes and table pointers it was captured with.
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    size_t capCostLds = 0, capWta0Lds = 0, capWta1Lds = 0;
+    const void* capPtrs[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool capBig = false, capSpeckle = false;
+    float times[4] = {0, 0, 0, 0};  // total, upload, compute, download (ms)
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE;
+    }
+    ~lumen_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        void* dev[] = {arena, jobs, items, itemsBig, tables, ctr, trash};
+        for (void* p : dev)
+            if (p) (void)hipFree(p);
+        if (hostIn) (void)hipHostFree(hostIn);
+        if (hostOut) (void)hipHostFree(hostOut);
+    }
+};
+
+namespace {
+
+template <typename T>
+hipError_t grow(T*& p, size_t& cap, size_t need) {
+    if (need <= cap && p) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    const size_t n = std::max<size_t>(need, 1) * 5 / 4 + 16;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), n * sizeof(T));
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+hipError_t grow_pinned(void** p, size_t& cap, size_t need) {
+    if (need <= cap && *p) return hipSuccess;
+    if (*p) {
+        (void)hipHostFree(*p);
+        *p = nullptr;
+        cap = 0;
+    }
+    const size_t n = need * 5 / 4 + 4096;
+    hipError_t e = hipHostMalloc(p, n, hipHostMallocDefault);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+// Host-side plan of one sub-batch.
+struct Plan {
+    std::vector<PairJob> jobs;
+    std::vector<PathItem> items, itemsBig;
+    std::vector<int> costStart, wta0Start, wta1Start, spkStart, medStart;
+    std::vector<int> wta0Pairs, wta1Pairs, spkPairs;
+    std::vector<int> special;  // pairs whose cost needs the per-pair kernels
+    size_t costLds = 0, wta0Lds = 0, wta1Lds = 0;
+};
+
+size_t wta_lds_for(int K, int W) {
+    switch (K) {
+        case 4: return wta_lds_bytes<4>(W);
+        case 8: return wta_lds_bytes<8>(W);
+        case 12: return wta_lds_bytes<12>(W);
+        case 16: return wta_lds_bytes<16>(W);
+        case 20: return wta_lds_bytes<20>(W);
+        case 24: return wta_lds_bytes<24>(W);
+        case 28: return wta_lds_bytes<28>(W);
+        default: return wta_lds_bytes<32>(W);
+    }
+}
+
+void build_plan(Plan& pl) {
+    const int n = static_cast<int>(pl.jobs.size());
+    pl.costStart.assign(1, 0);
+    pl.wta0Start.assign(1, 0);
+    pl.wta1Start.assign(1, 0);
+    pl.spkStart.assign(1, 0);
+    pl.medStart.assign(1, 0);
+    for (int p = 0; p < n; ++p) {
+        const PairJob& j = pl.jobs[p];
+        int cost = 0;
+        const int r = j.blockSize / 2;
+        if (j.mode == 1) {
+            cost = ((j.W + kCensusTile - 1) / kCensusTile) * j.H;
+            pl.costLds = std::max(pl.costLds, census_lds_bytes(j.Dp));
+        } else if (r >= 0 && r <= 31) {
+            cost = ((j.W + kSadTile - 1) / kSadTile) * j.H;
+            pl.costLds = std::max(pl.costLds, sad_lds_bytes(r, j.Dp));
+        } else {
+            pl.special.push_back(p);
+        }
+        pl.costStart.push_back(pl.costStart.back() + cost);
+        const bool big = j.K > 12;
+        if (big) {
+            pl.wta1Pairs.push_back(p);
+            pl.wta1Start.push_back(pl.wta1Start.back() + j.H);
+            pl.wta1Lds = std::max(pl.wta1Lds, wta_lds_for(j.K, j.W));
+        } else {
+            pl.wta0Pairs.push_back(p);
+            pl.wta0Start.push_back(pl.wta0Start.back() + j.H);
+            pl.wta0Lds = std::max(pl.wta0Lds, wta_lds_for(j.K, j.W));
+        }
+        if (j.doSpeckle) {
+            pl.spkPairs.push_back(p);
+            pl.spkStart.push_back(pl.spkStart.back() + ((j.W + 15) / 16) * ((j.H + 15) / 16));
+        }
+        pl.medStart.push_back(pl.medStart.back() +
+                              ((j.W + kMedTW - 1) / kMedTW) * ((j.H + kMedTH - 1) / kMedTH));
+        for (int path = 0; path < j.npaths; ++path) {
+            const int count = scan_count(path, j.W, j.H);
+            for (int s0 = 0; s0 < count; s0 += 4) {
+                PathItem it{static_cast<uint16_t>(p), static_cast<uint8_t>(path),
+                            static_cast<uint8_t>(j.K), s0};
+                (big ? pl.itemsBig : pl.items).push_back(it);
+            }
+        }
+    }
+    // Longest scanlines first (by the wave's first scanline, the longest of its four except at
+    // the diagonal corner), so the stage does not end on a long wave started late.
+    auto byLen = [&](const PathItem& a, const PathItem& b) {
+        const PairJob& ja = pl.jobs[a.pair];
+        const PairJob& jb = pl.jobs[b.pair];
+        const int la = scan_length(a.path, a.s0, ja.W, ja.H) * ja.K;
+        const int lb = scan_length(b.path, b.s0, jb.W, jb.H) * jb.K;
+        return la > lb;
+    };
+    std::stable_sort(pl.items.begin(), pl.items.end(), byLen);
+    std::stable_sort(pl.itemsBig.begin(), pl.itemsBig.end(), byLen);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lumen_batch_create(size_t arena_bytes, lumen_batch** out) {
+    if (!out) return LUMEN_EINVAL;
+    *out = nullptr;
+    lumen_batch* b = new (std::nothrow) lumen_batch();
+    if (!b) return LUMEN_ENOMEM;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&b->ev[i]);
+    if (e == hipSuccess && arena_bytes == 0) {
+        size_t freeB = 0, totalB = 0;
+        e = hipMemGetInfo(&freeB, &totalB);
+        arena_bytes = freeB / 2;
+    }
+    if (e == hipSuccess) e = hipMalloc(&b->arena, arena_bytes);
+    if (e == hipSuccess) b->arenaBytes = arena_bytes;
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->ctr), kNumCounters * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->trash), kTrashElems * 2);
+    if (e != hipSuccess) {
+        g_batch_error = std::string("lumen_batch_create: ") + hipGetErrorString(e);
+        delete b;
+        return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE;
+    }
+    b->ready = true;
+    *out = b;
+    return LUMEN_OK;
+}
+
+int lumen_batch_destroy(lumen_batch* b) {
+    delete b;
+    return LUMEN_OK;
+}
+
+const char* lumen_batch_last_error(const lumen_batch* b) {
+    return b ? b->err.c_str() : g_batch_error.c_str();
+}
+
+int lumen_batch_get_timings(const lumen_batch* b, float out[4]) {
+    if (!b || !out) return LUMEN_EINVAL;
+    for (int i = 0; i < 4; ++i) out[i] = b->times[i];
+    return LUMEN_OK;
+}
+
+int lumen_batch_arena_bytes(const lumen_batch* b, size_t* out) {
+    if (!b || !out) return LUMEN_EINVAL;
+    *out = b->arenaBytes;
+    return LUMEN_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+Tables tables_of(const lumen_batch* b) {
+    const int cap = static_cast<int>(b->pairCap);
+    Tables t;
+    t.hdr = b->tables;
+    t.costStart = t.hdr + kHdrFields;
+    t.wta0Start = t.costStart + (cap + 1);
+    t.wta1Start = t.wta0Start + (cap + 1);
+    t.spkStart = t.wta1Start + (cap + 1);
+    t.medStart = t.spkStart + (cap + 1);
+    t.wta0Pairs = t.medStart + (cap + 1);
+    t.wta1Pairs = t.wta0Pairs + cap;
+    t.spkPairs = t.wta1Pairs + cap;
+    return t;
+}
+
+size_t tables_ints(size_t cap) { return kHdrFields + 5 * (cap + 1) + 3 * cap; }
+
+// Captures (or reuses) the graph of the persistent pipeline. A captured graph stays valid while
+// the device tables keep their addresses and the batch needs no more LDS than it was captured
+// with; the batch's sizes are read from the device header at run time.
+int ensure_graph(lumen_batch* b, const Plan& pl) {
+    const bool big = !pl.itemsBig.empty() || !pl.wta1Pairs.empty();
+    const bool spk = !pl.spkPairs.empty();
+    const void* ptrs[4] = {b->jobs, b->items, b->itemsBig, b->tables};
+    if (b->exec && b->capCostLds >= pl.costLds && b->capWta0Lds >= pl.wta0Lds &&
+        b->capWta1Lds >= pl.wta1Lds && (b->capBig || !big) && (b->capSpeckle || !spk) &&
+        std::equal(ptrs, ptrs + 4, b->capPtrs))
+        return LUMEN_OK;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    const Tables t = tables_of(b);
+    const size_t costLds = std::max<size_t>(std::max(pl.costLds, b->capCostLds), 16);
+    const size_t wta0Lds = std::max<size_t>(std::max(pl.wta0Lds, b->capWta0Lds), 16);
+    const size_t wta1Lds = std::max<size_t>(std::max(pl.wta1Lds, b->capWta1Lds), 16);
+    const bool withBig = big || b->capBig;
+    const bool withSpk = spk || b->capSpeckle;
+    const int gCost = occupancy_blocks(batch_cost_kernel, costLds);
+    const int gPaths = occupancy_blocks(batch_paths_kernel<0>, 0);
+    const int gPathsBig = occupancy_blocks(batch_paths_kernel<1>, 0);
+    const int gWta0 = occupancy_blocks(batch_wta_kernel<0>, wta0Lds);
+    const int gWta1 = occupancy_blocks(batch_wta_kernel<1>, wta1Lds);
+    const int gSpk = occupancy_blocks(batch_speckle_kernel, 0);
+    const int gMed = occupancy_blocks(batch_median_kernel, 0);
+    hipStream_t s = b->stream;
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return b->hip_fail(e, "begin capture");
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, b->ctr);
+    hipLaunchKernelGGL(batch_cost_kernel, dim3(gCost), dim3(256), costLds, s, b->jobs, t, b->ctr);
+    hipLaunchKernelGGL(batch_paths_kernel<0>, dim3(gPaths), dim3(256), 0, s, b->jobs, b->items, t,
+                       b->ctr, b->trash);
+    if (withBig)
+        hipLaunchKernelGGL(batch_paths_kernel<1>, dim3(gPathsBig), dim3(256), 0, s, b->jobs,
+                           b->itemsBig, t, b->ctr, b->trash);
+    hipLaunchKernelGGL(batch_wta_kernel<0>, dim3(gWta0), dim3(256), wta0Lds, s, b->jobs, t, b->ctr);
+    if (withBig)
+        hipLaunchKernelGGL(batch_wta_kernel<1>, dim3(gWta1), dim3(256), wta1Lds, s, b->jobs, t, b->ctr);
+    if (withSpk)
+        hipLaunchKernelGGL(batch_speckle_kernel, dim3(gSpk), dim3(256), 0, s, b->jobs, t, b->ctr);
+    hipLaunchKernelGGL(batch_median_kernel, dim3(gMed), dim3(256), 0, s, b->jobs, t, b->ctr);
+    hipError_t le = hipGetLastError();
+    e = hipStreamEndCapture(s, &b->graph);
+    if (le != hipSuccess) return b->hip_fail(le, "captured launch");
+    if (e != hipSuccess) return b->hip_fail(e, "end capture");
+    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return b->hip_fail(e, "graph instantiate");
+    b->capCostLds = costLds;
+    b->capWta0Lds = wta0Lds;
+    b->capWta1Lds = wta1Lds;
+    b->capBig = withBig;
+    b->capSpeckle = withSpk;
+    std::copy(ptrs, ptrs + 4, b->capPtrs);
+    return LUMEN_OK;
+}
+
+// Runs one sub-batch whose jobs already point at device images and outputs.
+int run_plan(lumen_batch* b, Plan& pl) {
+    const int npairs = static_cast<int>(pl.jobs.size());
+    build_plan(pl);
+    hipStream_t s = b->stream;
+    // Tables grow (and move, which forces a new capture) only when a batch outgrows them.
+    hipError_t e = grow(b->jobs, b->jobsCap, npairs);
+    if (e == hipSuccess) e = grow(b->items, b->itemsCap, pl.items.size());
+    if (e == hipSuccess) e = grow(b->itemsBig, b->itemsBigCap, pl.itemsBig.size());
+    if (e == hipSuccess && (size_t)npairs > b->pairCap) {
+        if (b->tables) (void)hipFree(b->tables);
+        b->tables = nullptr;
+        const size_t cap = std::max<size_t>(npairs, 64);
+        e = hipMalloc(reinterpret_cast<void**>(&b->tables), tables_ints(cap) * sizeof(int));
+        b->pairCap = e == hipSuccess ? cap : 0;
+    }
+    if (e != hipSuccess) return b->hip_fail(e, "batch tables");
+    const int cap = static_cast<int>(b->pairCap);
+    std::vector<int> tab(tables_ints(cap), 0);
+    tab[kHdrPairs] = npairs;
+    tab[kHdrWta0] = static_cast<int>(pl.wta0Pairs.size());
+    tab[kHdrWta1] = static_cast<int>(pl.wta1Pairs.size());
+    tab[kHdrSpeckle] = static_cast<int>(pl.spkPairs.size());
+    tab[kHdrItems] = static_cast<int>(pl.items.size());
+    tab[kHdrItemsBig] = static_cast<int>(pl.itemsBig.size());
+    int* base = tab.data() + kHdrFields;
+    auto put = [&](const std::vector<int>& v, int n) {
+        for (int i = 0; i < n; ++i) base[i] = i < (int)v.size() ? v[i] : (v.empty() ? 0 : v.back());
+        base += n;
+    };
+    put(pl.costStart, cap + 1);
+    put(pl.wta0Start, cap + 1);
+    put(pl.wta1Start, cap + 1);
+    put(pl.spkStart, cap + 1);
+    put(pl.medStart, cap + 1);
+    put(pl.wta0Pairs, cap);
+    put(pl.wta1Pairs, cap);
+    put(pl.spkPairs, cap);
+    e = hipMemcpyAsync(b->jobs, pl.jobs.data(), npairs * sizeof(PairJob), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !pl.items.empty())
+        e = hipMemcpyAsync(b->items, pl.items.data(), pl.items.size() * sizeof(PathItem),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !pl.itemsBig.empty())
+        e = hipMemcpyAsync(b->itemsBig, pl.itemsBig.data(), pl.itemsBig.size() * sizeof(PathItem),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(b->tables, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return b->hip_fail(e, "table upload");
+    // Pairs whose SAD window is outside the LDS kernel's range get per-pair cost launches.
+    for (int p : pl.special) {
+        e = launch_cost(pl.jobs[p], s);
+        if (e != hipSuccess) return b->hip_fail(e, "per-pair cost launch");
+    }
+    int rc = ensure_graph(b, pl);
+    if (rc) return rc;
+    e = hipGraphLaunch(b->exec, s);
+    if (e != hipSuccess) return b->hip_fail(e, "graph launch");
+    return LUMEN_OK;
+}
+
+}  // namespace
+
+extern "C" int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs,
+                               int device_io) {
+    if (!b) return LUMEN_EINVAL;
+    b->err.clear();
+    if (npairs < 0 || (npairs > 0 && !pairs)) return b->fail(LUMEN_EINVAL, "no pairs");
+    if (npairs > 65535) return b->fail(LUMEN_EINVAL, "at most 65535 pairs per batch");
+    for (int i = 0; i < npairs; ++i) {
+        const lumen_pair_desc& d = pairs[i];
+        const std::string msg = validate_pair(d.params, d.width, d.height, d.left && d.right);
+        if (!msg.empty()) return b->fail(LUMEN_EINVAL, "pair " + std::to_string(i) + ": " + msg);
+        if (!d.out) return b->fail(LUMEN_EINVAL, "pair " + std::to_string(i) + ": output is NULL");
+        if (pair_footprint(d.params, d.width, d.height) + 3 * 256 > b->arenaBytes)
+            return b->fail(LUMEN_ENOMEM, "pair " + std::to_string(i) + " does not fit the arena");
+    }
+    hipStream_t s = b->stream;
+    hipError_t e = hipEventRecord(b->ev[0], s);
+    if (e != hipSuccess) return b->hip_fail(e, "event");
+    float tUp = 0, tComp = 0, tDown = 0;
+    int first = 0;
+    while (first < npairs) {
+        // Greedy sub-batch: consecutive pairs while their working sets fit the arena.
+        Plan pl;
+        size_t used = 0;
+        int last = first;
+        std::vector<size_t> inOff, outOff;
+        size_t inBytes = 0, outBytes = 0;
+        char* base = static_cast<char*>(b->arena);
+        while (last < npairs) {
+            const lumen_pair_desc& d = pairs[last];
+            const size_t n = (size_t)d.width * d.height;
+            const size_t need = align_up(pair_footprint(d.params, d.width, d.height), 256) + 8 * 256;
+            if (used + need > b->arenaBytes) break;
+            PairJob j{};
+            fill_job(j, d.params, d.width, d.height);
+            auto take = [&](size_t bytes) {
+                char* p = base + used;
+                used += align_up(bytes, 256);
+                return p;
+            };
+            uint8_t* dl = reinterpret_cast<uint8_t*>(take(n));
+            uint8_t* dr = reinterpret_cast<uint8_t*>(take(n));
+            j.left = device_io ? d.left : dl;
+            j.right = device_io ? d.right : dr;
+            j.cost = reinterpret_cast<uint8_t*>(take(j.pathStride));
+            j.L = reinterpret_cast<uint16_t*>(take(j.pathStride * 2 * j.npaths));
+            j.dispL = reinterpret_cast<int16_t*>(take(n * 2));
+            j.dispR = reinterpret_cast<int16_t*>(take(n * 2));
+            j.dispLR = reinterpret_cast<int16_t*>(take(n * 2));
+            j.wtaInt = reinterpret_cast<int16_t*>(take(n * 2));
+            j.spk = reinterpret_cast<int16_t*>(take(n * 2));
+            int16_t* dout = reinterpret_cast<int16_t*>(take(n * 2));
+            j.out = device_io ? d.out : dout;
+            pl.jobs.push_back(j);
+            inOff.push_back(inBytes);
+            inBytes += 2 * n;
+            outOff.push_back(outBytes);
+            outBytes += n * 2;
+            ++last;
+        }
+        if (last == first) return b->fail(LUMEN_ENOMEM, "pair does not fit the arena");
+        const int cnt = last - first;
+        if (!device_io) {
+            // Host images -> pinned staging -> device, one copy per image.
+            e = grow_pinned(reinterpret_cast<void**>(&b->hostIn), b->hostInCap, inBytes);
+            if (e == hipSuccess) e = grow_pinned(reinterpret_cast<void**>(&b->hostOut), b->hostOutCap, outBytes);
+            if (e != hipSuccess) return b->hip_fail(e, "pinned staging");
+            e = hipStreamSynchronize(s);  // staging is reused between sub-batches
+            if (e != hipSuccess) return b->hip_fail(e, "sync");
+            for (int i = 0; i < cnt; ++i) {
+                const lumen_pair_desc& d = pairs[first + i];
+                const size_t n = (size_t)d.width * d.height;
+                std::memcpy(b->hostIn + inOff[i], d.left, n);
+                std::memcpy(b->hostIn + inOff[i] + n, d.right, n);
+            }
+            e = hipEventRecord(b->ev[1], s);
+            for (int i = 0; e == hipSuccess && i < cnt; ++i) {
+                const size_t n = (size_t)pl.jobs[i].W * pl.jobs[i].H;
+                e = hipMemcpyAsync(const_cast<uint8_t*>(pl.jobs[i].left), b->hostIn + inOff[i], n,
+                                   hipMemcpyHostToDevice, s);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(const_cast<uint8_t*>(pl.jobs[i].right), b->hostIn + inOff[i] + n,
+                                       n, hipMemcpyHostToDevice, s);
+            }
+            if (e != hipSuccess) return b->hip_fail(e, "upload");
+        } else {
+            e = hipEventRecord(b->ev[1], s);
+            if (e != hipSuccess) return b->hip_fail(e, "event");
+        }
+        e = hipEventRecord(b->ev[2], s);
+        if (e != hipSuccess) return b->hip_fail(e, "event");
+        int rc = run_plan(b, pl);
+        if (rc) return rc;
+        e = hipEventRecord(b->ev[3], s);
+        if (e != hipSuccess) return b->hip_fail(e, "event");
+        if (!device_io) {
+            for (int i = 0; e == hipSuccess && i < cnt; ++i) {
+                const size_t n = (size_t)pl.jobs[i].W * pl.jobs[i].H;
+                e = hipMemcpyAsync(b->hostOut + outOff[i] / 2, pl.jobs[i].out, n * 2,
+                                   hipMemcpyDeviceToHost, s);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return b->hip_fail(e, "download");
+            for (int i = 0; i < cnt; ++i) {
+                const lumen_pair_desc& d = pairs[first + i];
+                std::memcpy(d.out, b->hostOut + outOff[i] / 2, (size_t)d.width * d.height * 2);
+            }
+        } else {
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return b->hip_fail(e, "sync");
+        }
+        float up = 0, comp = 0;
+        if (hipEventElapsedTime(&up, b->ev[1], b->ev[2]) == hipSuccess) tUp += up;
+        if (hipEventElapsedTime(&comp, b->ev[2], b->ev[3]) == hipSuccess) tComp += comp;
+        first = last;
+    }
+    float total = 0;
+    e = hipEventRecord(b->ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(b->ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&total, b->ev[0], b->ev[1]);
+    if (e != hipSuccess) return b->hip_fail(e, "timing");
+    tDown = std::max(0.0f, total - tUp - tComp);
+    b->times[0] = total;
+    b->times[1] = tUp;
+    b->times[2] = tComp;
+    b->times[3] = tDown;
+    return LUMEN_OK;
+}

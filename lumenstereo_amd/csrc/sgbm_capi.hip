@@ -7,6 +7,7 @@
 // too). Nothing is zeroed per call: every buffer a stage reads was fully written by the stage
 // before it.
 #include "lumen_sgbm.h"
+#include "sgbm_host.h"
 #include "sgbm_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -60,14 +61,6 @@ struct Geometry {
     size_t pixels() const { return (size_t)W * H; }
     size_t volume() const { return pixels() * Dp; }
 };
-
-int num_paths(const lumen_stereo_params& p) { return p.numDirections >= 8 ? 8 : 4; }
-
-// src/sgbm_gpu.cu:312-317.
-int cost_scale(const lumen_stereo_params& p) {
-    if (p.matchingCostMode == 0) return p.blockSize * p.blockSize;
-    return std::max(1, (p.blockSize * p.blockSize * 2 * p.preFilterCap) / 255);
-}
 
 }  // namespace
 
@@ -125,23 +118,8 @@ namespace {
 
 // Validation of src/sgbm_gpu.cu:148-166, plus the limits of this implementation.
 int validate(lumen_sgbm* h, const void* left, const void* right, int W, int H) {
-    if (!left || !right || W <= 0 || H <= 0) return h->fail(LUMEN_EINVAL, "Input images are empty");
-    const lumen_stereo_params& p = h->params;
-    const int D = p.maxDisparity - p.minDisparity;
-    if (D < 1)
-        return h->fail(LUMEN_EINVAL,
-                       "Invalid disparity range: maxDisparity must be greater than minDisparity");
-    if (D > LUMEN_MAX_DISPARITY_RANGE)
-        return h->fail(LUMEN_EINVAL,
-                       "Disparity range (maxDisparity - minDisparity) exceeds GPU limit (" +
-                           std::to_string(LUMEN_MAX_DISPARITY_RANGE) + "). Reduce the search range.");
-    if (p.minDisparity < 0)
-        return h->fail(LUMEN_EINVAL, "minDisparity must be >= 0");
-    if (p.matchingCostMode != 1 && W < 2)
-        return h->fail(LUMEN_EINVAL, "Image width must be at least 2 for the SAD prefilter");
-    if (W > kMaxWidth)
-        return h->fail(LUMEN_EINVAL, "Image width exceeds " + std::to_string(kMaxWidth));
-    return LUMEN_OK;
+    const std::string msg = validate_pair(h->params, W, H, left != nullptr && right != nullptr);
+    return msg.empty() ? LUMEN_OK : h->fail(LUMEN_EINVAL, msg);
 }
 
 int prepare(lumen_sgbm* h, int W, int H) {
@@ -186,73 +164,33 @@ int prepare(lumen_sgbm* h, int W, int H) {
 // caller's device buffers.
 int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dout, int slot) {
     const Geometry& g = h->geo;
-    const lumen_stereo_params& p = h->params;
     hipStream_t s = h->stream;
-    hipError_t e;
-    if (p.matchingCostMode == 1) {
-        e = launch_cost_census(dl, dr, h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, s);
-        if (e != hipSuccess) return h->hip_fail(e, "census cost launch");
-    } else {
-        e = launch_cost_sad(dl, dr, h->cost.as<uint8_t>(), g.W, g.H, g.D, g.Dp, g.m, p.blockSize,
-                            p.preFilterCap, s);
-        if (e != hipSuccess) return h->hip_fail(e, "SAD cost launch");
-    }
+    PairJob j{};
+    fill_job(j, h->params, g.W, g.H);
+    j.left = dl;
+    j.right = dr;
+    j.cost = h->cost.as<uint8_t>();
+    j.L = h->paths.as<uint16_t>();
+    j.dispL = h->dispL.as<int16_t>();
+    j.dispR = h->dispR.as<int16_t>();
+    j.dispLR = h->dispLR.as<int16_t>();
+    j.wtaInt = h->wtaInt.as<int16_t>();
+    j.spk = h->spk.as<int16_t>();
+    j.out = dout;
+    hipError_t e = launch_cost(j, s);
+    if (e != hipSuccess) return h->hip_fail(e, "cost launch");
     e = hipEventRecord(h->ev[slot][2], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
-
-    PathsArgs pa{};
-    pa.cost = h->cost.as<uint8_t>();
-    pa.img = dl;
-    pa.L = h->paths.as<uint16_t>();
-    pa.trash = h->trash.as<uint16_t>();
-    pa.pathStride = g.volume();
-    pa.W = g.W;
-    pa.H = g.H;
-    pa.D = g.D;
-    pa.Dp = g.Dp;
-    pa.P1 = p.P1;
-    pa.costScale = g.costScale;
-    pa.npaths = g.npaths;
-    // t16(P2adj) for every |dI| (src/sgbm_kernels.cu:304-310).
-    for (int gd = 0; gd < 256; ++gd) {
-        const int p2 = std::max(p.P1 + 1, p.P2 * 8 / std::max(8, gd));
-        pa.p2tab[gd] = static_cast<uint16_t>(static_cast<unsigned>(p2) & 0xFFFFu);
-    }
-    e = launch_paths(pa, s);
+    e = launch_paths(j, h->trash.as<uint16_t>(), s);
     if (e != hipSuccess) return h->hip_fail(e, "path aggregation launch");
     e = hipEventRecord(h->ev[slot][3], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
-
-    WtaArgs wa{};
-    wa.L = h->paths.as<uint16_t>();
-    wa.pathStride = g.volume();
-    wa.npaths = g.npaths;
-    wa.W = g.W;
-    wa.H = g.H;
-    wa.D = g.D;
-    wa.Dp = g.Dp;
-    wa.m = g.m;
-    wa.uniquenessRatio = p.uniquenessRatio;
-    wa.doLR = p.disp12MaxDiff >= 0 ? 1 : 0;
-    wa.lrMaxDiff = p.disp12MaxDiff * 16;
-    wa.dispL = h->dispL.as<int16_t>();
-    wa.dispR = h->dispR.as<int16_t>();
-    wa.dispLR = h->dispLR.as<int16_t>();
-    wa.wtaInt = h->wtaInt.as<int16_t>();
-    e = launch_wta(wa, s);
+    e = launch_wta(j, s);
     if (e != hipSuccess) return h->hip_fail(e, "WTA launch");
     e = hipEventRecord(h->ev[slot][4], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
-
-    const int16_t* medIn = h->dispLR.as<int16_t>();
-    if (p.speckleWindowSize > 0) {
-        e = launch_speckle(h->dispLR.as<int16_t>(), h->spk.as<int16_t>(), g.W, g.H,
-                           std::min(p.speckleWindowSize, 25), p.speckleRange * 16, s);
-        if (e != hipSuccess) return h->hip_fail(e, "speckle launch");
-        medIn = h->spk.as<int16_t>();
-    }
-    e = launch_median(medIn, dout, g.W, g.H, s);
-    if (e != hipSuccess) return h->hip_fail(e, "median launch");
+    e = launch_post(j, s);
+    if (e != hipSuccess) return h->hip_fail(e, "post-processing launch");
     e = hipEventRecord(h->ev[slot][5], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
     return LUMEN_OK;
@@ -355,10 +293,7 @@ int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int heigh
     if (!params || !bytes_out || width <= 0 || height <= 0) return LUMEN_EINVAL;
     const int D = params->maxDisparity - params->minDisparity;
     if (D < 1 || D > LUMEN_MAX_DISPARITY_RANGE) return LUMEN_EINVAL;
-    const size_t n = (size_t)width * height;
-    const size_t vol = n * padded_range(D);
-    size_t b = vol + vol * 2 * num_paths(*params) + n * 2 + n * 2 * 6;
-    *bytes_out = b;
+    *bytes_out = pair_footprint(*params, width, height);
     return LUMEN_OK;
 }
 

@@ -1,0 +1,898 @@
+// sgbm_device.h - device code of the matcher, shared by the per-pair kernels
+// (sgbm_kernels.hip) and the persistent batched kernels (sgbm_batch.hip).
+//
+// Every stage is a __device__ body over one unit of work of one pair (a row segment for the
+// cost, a wave of four scanlines for the aggregation, a row for the WTA, a tile for the
+// post-processing); the per-pair and batched kernels differ only in how they hand out those
+// units, so their results are identical by construction.
+//
+// Layouts (all in HBM, row-major):
+//   images            uint8  [H][W]
+//   cost volume C     uint8  [H][W][Dp]   disparity innermost, Dp = D rounded up to 64
+//   path costs L_r    uint16 [npaths][H][W][Dp]
+//   disparity maps    int16  [H][W]       1/16 px, -1 invalid
+// Padding entries d >= D carry cost 255 and are forced to 65535 in every path cost.
+//
+// Every integer operation reproduces the reference's arithmetic exactly, including the
+// truncations to uint16_t (src/sgbm_kernels.cu:338-360); packed 16-bit VALU operations wrap
+// mod 2^16 as those truncations do.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "sgbm_internal.h"
+
+namespace lumen {
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t bits(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ u16x2 as16(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ u16x2 splat(uint32_t v) { return as16((v & 0xFFFFu) * 0x10001u); }
+__device__ __forceinline__ u16x2 vmin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
+
+// DPP controls (GFX9 encoding).
+constexpr int kQuadXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kQuadXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kRowShl1 = 0x101;        // lane i reads lane i+1 of its row
+constexpr int kRowShr1 = 0x111;        // lane i reads lane i-1 of its row
+constexpr int kRowMirror = 0x140;      // lane i reads lane 15-i of its row
+constexpr int kRowHalfMirror = 0x141;  // lane i reads lane 7-i of its half row
+
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v, uint32_t old) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old),
+                                                             static_cast<int>(v), CTRL, 0xF, 0xF,
+                                                             false));
+}
+
+// Minimum over the 16 lanes of a DPP row, left in every lane of the row.
+__device__ __forceinline__ uint32_t row_min(uint32_t v) {
+    v = min(v, dpp<kQuadXor1>(v, v));
+    v = min(v, dpp<kQuadXor2>(v, v));
+    v = min(v, dpp<kRowHalfMirror>(v, v));
+    v = min(v, dpp<kRowMirror>(v, v));
+    return v;
+}
+
+// Value of lane u of the lane's DPP row (row_newbcast, gfx90a and later); u is folded to a
+// constant after unrolling.
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v, int u) {
+    switch (u) {
+        case 0: return dpp<0x150>(v, v);
+        case 1: return dpp<0x151>(v, v);
+        case 2: return dpp<0x152>(v, v);
+        case 3: return dpp<0x153>(v, v);
+        case 4: return dpp<0x154>(v, v);
+        case 5: return dpp<0x155>(v, v);
+        case 6: return dpp<0x156>(v, v);
+        case 7: return dpp<0x157>(v, v);
+        case 8: return dpp<0x158>(v, v);
+        case 9: return dpp<0x159>(v, v);
+        case 10: return dpp<0x15A>(v, v);
+        case 11: return dpp<0x15B>(v, v);
+        case 12: return dpp<0x15C>(v, v);
+        case 13: return dpp<0x15D>(v, v);
+        case 14: return dpp<0x15E>(v, v);
+        default: return dpp<0x15F>(v, v);
+    }
+}
+
+// s_waitcnt vmcnt(N) with expcnt and lgkmcnt left open (GFX9 encoding).
+#define LUMEN_WAIT_VMCNT(N) \
+    __builtin_amdgcn_s_waitcnt(((N) & 0xF) | (7 << 4) | (15 << 8) | ((((N) >> 4) & 3) << 14))
+
+// float -> int16 as the reference's device conversion: truncate, clamp to the int16 range.
+__device__ __forceinline__ int16_t f2i16(float v) {
+    float t = truncf(v);
+    t = fminf(fmaxf(t, -32768.0f), 32767.0f);
+    return static_cast<int16_t>(static_cast<int>(t));
+}
+
+// Parabola fit of src/sgbm_kernels.cu:789-804 / :836-855 in IEEE single precision.
+__device__ __forceinline__ float parabola(float s, uint32_t a0, uint32_t a1, uint32_t a2) {
+    const float c0 = static_cast<float>(a0);
+    const float c1 = static_cast<float>(a1);
+    const float c2 = static_cast<float>(a2);
+    const float denom = 2.0f * (c0 + c2 - 2.0f * c1);
+    if (fabsf(denom) > 1e-6f) {
+        float delta = (c0 - c2) / denom;
+        delta = fmaxf(-1.0f, fminf(1.0f, delta));
+        s += delta;
+    }
+    return s;
+}
+
+// ---------------------------------------------------------------------------------------
+// Census cost (src/sgbm_kernels.cu:217-286), census transform fused in: one row segment of
+// CT pixels per call, block-wide. The five image rows around the row are staged in LDS, the
+// 24-bit census codes of the segment and of every right pixel it can reach are computed into
+// LDS (bit k over the 5x5 neighbours in row-major order, centre skipped, set when the
+// neighbour is >= the centre; code 0 within 2 px of the border), then each thread writes 16
+// consecutive disparities of one pixel as one 16-byte store.
+// ---------------------------------------------------------------------------------------
+constexpr int kCensusTile = 256;  // pixels per census cost segment
+constexpr int kSadTile = 64;      // pixels per SAD cost segment
+
+__host__ __device__ inline size_t census_lds_bytes(int Dp) {
+    const int nr = kCensusTile + Dp - 1;
+    return (size_t)(kCensusTile + nr) * 4 + 5 * (kCensusTile + 4) + 5 * (nr + 4);
+}
+
+__host__ __device__ inline size_t sad_lds_bytes(int r, int Dp) {
+    return (size_t)(2 * r + 1) * ((kSadTile + 2 * r) + (kSadTile + 2 * r + Dp - 1));
+}
+
+__device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, int c) {
+    // rows: 5 staged image rows, column c is the centre's column in the staging buffer.
+    const uint8_t ctr = rows[2 * pitch + c];
+    uint32_t code = 0;
+    int k = 0;
+#pragma unroll
+    for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+        for (int dx = -2; dx <= 2; ++dx) {
+            if (dy == 2 && dx == 0) continue;
+            code |= (rows[dy * pitch + c + dx] >= ctr ? 1u : 0u) << k;
+            ++k;
+        }
+    return code;
+}
+
+// Block-wide; every thread of the block must call it with the same (y, x0).
+__device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* shc) {
+    constexpr int T = kCensusTile;
+    const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int rs = x0 - m - (Dp - 1);  // first right pixel the segment can reach
+    const int nr = T + Dp - 1;         // right pixels the segment can reach
+    const int lpitch = T + 4, rpitch = nr + 4;
+    uint32_t* sl = shc;                                         // T codes
+    uint32_t* sr = shc + T;                                     // nr codes
+    uint8_t* il = reinterpret_cast<uint8_t*>(shc + T + nr);  // 5 x lpitch bytes
+    uint8_t* ir = il + 5 * lpitch;                              // 5 x rpitch bytes
+    const bool rowOk = y >= 2 && y < H - 2;
+    if (rowOk) {
+#pragma unroll
+        for (int r = 0; r < 5; ++r) {
+            const uint8_t* lrow = j.left + (size_t)(y - 2 + r) * W;
+            const uint8_t* rrow = j.right + (size_t)(y - 2 + r) * W;
+            for (int i = tid; i < lpitch; i += nt) {
+                const int c = x0 - 2 + i;
+                il[r * lpitch + i] = (c >= 0 && c < W) ? lrow[c] : uint8_t(0);
+            }
+            for (int i = tid; i < rpitch; i += nt) {
+                const int c = rs - 2 + i;
+                ir[r * rpitch + i] = (c >= 0 && c < W) ? rrow[c] : uint8_t(0);
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < T; i += nt) {
+            const int x = x0 + i;
+            sl[i] = (x >= 2 && x < W - 2) ? census_at(il, lpitch, i + 2) : 0u;
+        }
+        for (int i = tid; i < nr; i += nt) {
+            const int rx = rs + i;
+            sr[i] = (rx >= 2 && rx < W - 2) ? census_at(ir, rpitch, i + 2) : 0u;
+        }
+        __syncthreads();
+    }
+    const int chunks = Dp / 16;
+    // item = pixel * chunks + chunk, advanced by nt items without a division per item.
+    int i = tid / chunks, c = tid - i * chunks;
+    const int di = nt / chunks, dc = nt - di * chunks;
+    for (; i < T; i += di, c += dc) {
+        if (c >= chunks) {
+            c -= chunks;
+            ++i;
+            if (i >= T) break;
+        }
+        const int x = x0 + i;
+        if (x >= W) break;
+        const bool leftOk = rowOk && x >= 2 && x < W - 2;
+        const uint32_t a = leftOk ? sl[i] : 0u;
+        const int d0 = c * 16;
+        const uint32_t* rc = sr + i + Dp - 1 - d0;  // rc[-k] is the right code for d0 + k
+        uint32_t w[4];
+        if (leftOk && d0 + 15 < D && x - m - d0 - 15 >= 2 && x - m - d0 < W - 2) {
+            // Interior: all 16 disparities valid. (h*255+12)/24 == (h*2718+168) >> 8 for
+            // h in [0,24]; two results per packed 16-bit multiply-add, byte 1 of each half
+            // gathered with one byte permute per four results.
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t h0 = __builtin_popcount(a ^ rc[-(4 * q + 0)]);
+                const uint32_t h1 = __builtin_popcount(a ^ rc[-(4 * q + 1)]);
+                const uint32_t h2 = __builtin_popcount(a ^ rc[-(4 * q + 2)]);
+                const uint32_t h3 = __builtin_popcount(a ^ rc[-(4 * q + 3)]);
+                const u16x2 k = splat(2718u), b = splat(168u);
+                const u16x2 v01 = as16(h0 | (h1 << 16)) * k + b;
+                const u16x2 v23 = as16(h2 | (h3 << 16)) * k + b;
+                w[q] = __builtin_amdgcn_perm(bits(v23), bits(v01), 0x07050301u);
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int d = d0 + q * 4 + b;
+                    const int rx = x - m - d;
+                    uint32_t v = 255;
+                    if (leftOk && d < D && rx >= 2 && rx < W - 2) {
+                        const uint32_t h = __builtin_popcount(a ^ rc[-(4 * q + b)]);
+                        v = (h * 255u + 12u) / 24u;
+                    }
+                    word |= v << (8 * b);
+                }
+                w[q] = word;
+            }
+        }
+        *reinterpret_cast<u32x4*>(j.cost + ((size_t)y * W + x) * Dp + d0) =
+            u32x4{w[0], w[1], w[2], w[3]};
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Sobel-x prefilter (src/sgbm_kernels.cu:38-59) fused into the SAD cost
+// (src/sgbm_kernels.cu:172-212): one row segment of kSadTile pixels per call, block-wide. The
+// prefiltered rows y-r..y+r of the segment and of the right columns it can reach are built in
+// LDS. The valid taps of a window form a rectangle, so count = rows * columns.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ uint8_t prefilter_at(const uint8_t* __restrict__ row, int c, int W,
+                                                int cap) {
+    int g;
+    if (c > 0 && c < W - 1)
+        g = int(row[c + 1]) - int(row[c - 1]);
+    else if (c == 0)
+        g = int(row[1]) - int(row[0]);
+    else
+        g = int(row[c]) - int(row[c - 1]);
+    g = min(cap, max(-cap, g));
+    return static_cast<uint8_t>(g + cap);
+}
+
+// Block-wide; every thread of the block must call it with the same (y, x0). Needs r <= 31.
+__device__ void sad_cost_segment(const PairJob& j, int y, int x0, uint8_t* shs) {
+    const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
+    const int r = j.blockSize / 2;
+    const int cap = j.preFilterCap > 1 ? j.preFilterCap : 1;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int lw = kSadTile + 2 * r;
+    const int rw = kSadTile + 2 * r + Dp - 1;
+    const int rows = 2 * r + 1;
+    uint8_t* fl = shs;
+    uint8_t* fr = shs + rows * lw;
+    const int ls = x0 - r, rs = x0 - r - m - (Dp - 1);
+    for (int k = 0; k < rows; ++k) {
+        const int yy = y - r + k;
+        const bool rowIn = yy >= 0 && yy < H;
+        for (int i = tid; i < lw; i += nt) {
+            const int c = ls + i;
+            fl[k * lw + i] = (rowIn && c >= 0 && c < W)
+                                 ? prefilter_at(j.left + (size_t)yy * W, c, W, cap) : uint8_t(0);
+        }
+        for (int i = tid; i < rw; i += nt) {
+            const int c = rs + i;
+            fr[k * rw + i] = (rowIn && c >= 0 && c < W)
+                                 ? prefilter_at(j.right + (size_t)yy * W, c, W, cap) : uint8_t(0);
+        }
+    }
+    __syncthreads();
+    const int dylo = max(-r, -y), dyhi = min(r, H - 1 - y);
+    const int nrows = dyhi - dylo + 1;
+    const int chunks = Dp / 16;
+    for (int item = tid; item < kSadTile * chunks; item += nt) {
+        const int i = item / chunks, c = item - i * chunks;
+        const int x = x0 + i;
+        if (x >= W) continue;
+        const int dxhi = min(r, W - 1 - x);
+        const int d0 = c * 16;
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint32_t word = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int d = d0 + q * 4 + b;
+                uint32_t v = 255;
+                if (d < D) {
+                    const int a = m + d;
+                    const int dxlo = max(-r, a - x);
+                    const int ncols = dxhi - dxlo + 1;
+                    if (ncols > 0 && nrows > 0) {
+                        int sum = 0;
+                        for (int dy = dylo; dy <= dyhi; ++dy) {
+                            const uint8_t* lrow = fl + (r + dy) * lw + i + r;
+                            const uint8_t* rrow = fr + (r + dy) * rw + i + r + Dp - 1 - d;
+                            for (int dx = dxlo; dx <= dxhi; ++dx)
+                                sum += abs(int(lrow[dx]) - int(rrow[dx]));
+                        }
+                        v = static_cast<uint32_t>(sum / (ncols * nrows));
+                    }
+                }
+                word |= (v & 0xFFu) << (8 * b);
+            }
+            w[q] = word;
+        }
+        *reinterpret_cast<u32x4*>(j.cost + ((size_t)y * W + x) * Dp + d0) =
+            u32x4{w[0], w[1], w[2], w[3]};
+    }
+}
+
+// SAD cost of one (pixel, disparity) read from the images directly (windows with r > 31).
+__device__ __forceinline__ uint8_t sad_cost_direct(const PairJob& j, int x, int y, int d) {
+    const int W = j.W, H = j.H;
+    const int r = j.blockSize / 2;
+    const int cap = j.preFilterCap > 1 ? j.preFilterCap : 1;
+    if (d >= j.D) return 255;
+    const int a = j.m + d;
+    const int dylo = max(-r, -y), dyhi = min(r, H - 1 - y);
+    const int dxlo = max(-r, a - x), dxhi = min(r, W - 1 - x);
+    if (dyhi < dylo || dxhi < dxlo) return 255;
+    int sum = 0;
+    for (int dy = dylo; dy <= dyhi; ++dy) {
+        const uint8_t* lr = j.left + (size_t)(y + dy) * W;
+        const uint8_t* rr = j.right + (size_t)(y + dy) * W;
+        for (int dx = dxlo; dx <= dxhi; ++dx)
+            sum += abs(int(prefilter_at(lr, x + dx, W, cap)) - int(prefilter_at(rr, x + dx - a, W, cap)));
+    }
+    return static_cast<uint8_t>(sum / ((dyhi - dylo + 1) * (dxhi - dxlo + 1)));
+}
+
+// ---------------------------------------------------------------------------------------
+// SGM aggregation, one wave = four neighbouring scanlines of one path.
+//
+// A scanline is the sequence of pixels of one path from its origin (the pixel whose
+// predecessor lies outside the image) to the far border. Each scanline is owned by one DPP row
+// (16 lanes); lane l holds disparities [l*K, l*K+K) as K/2 packed uint16 pairs, so the DP step
+// is packed 16-bit VALU work, the d-1 / d+1 neighbours cross lanes with one row_shr/row_shl
+// each, and min over d is a 4-step DPP butterfly. Scanline geometry is origin + t*step in
+// flat pixel index. Path order: 0 LR, 1 RL, 2 TB, 3 BT, 4 TLBR, 5 BRTL, 6 TRBL, 7 BLTR.
+// ---------------------------------------------------------------------------------------
+__host__ __device__ inline int scan_count(int path, int W, int H) {
+    return path < 2 ? H : (path < 4 ? W : W + H - 1);
+}
+
+// Length of scanline s of a path (0 when s is past the last scanline).
+__host__ __device__ inline int scan_length(int path, int s, int W, int H) {
+    if (s >= scan_count(path, W, H)) return 0;
+    if (path < 2) return W;
+    if (path < 4) return H;
+    return s < H - 1 ? min(W, s + 1) : min(W - (s - (H - 1)), H);
+}
+
+__device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0, int& step,
+                                          int& len) {
+    p0 = 0;
+    step = 0;
+    len = 0;
+    if (s >= scan_count(path, W, H)) return;
+    int x, y, dx, dy;
+    if (path < 2) {  // LR, RL
+        y = s;
+        x = path == 0 ? 0 : W - 1;
+        dx = path == 0 ? 1 : -1;
+        dy = 0;
+        len = W;
+    } else if (path < 4) {  // TB, BT
+        x = s;
+        y = path == 2 ? 0 : H - 1;
+        dx = 0;
+        dy = path == 2 ? 1 : -1;
+        len = H;
+    } else {  // diagonals, laid out in the TLBR frame and mirrored
+        int bx, by;
+        if (s < H - 1) {
+            bx = 0;
+            by = H - 1 - s;
+            len = min(W, s + 1);
+        } else {
+            bx = s - (H - 1);
+            by = 0;
+            len = min(W - bx, H);
+        }
+        const bool fx = (path == 5 || path == 6), fy = (path == 5 || path == 7);
+        x = fx ? W - 1 - bx : bx;
+        y = fy ? H - 1 - by : by;
+        dx = fx ? -1 : 1;
+        dy = fy ? -1 : 1;
+    }
+    p0 = y * W + x;
+    step = dy * W + dx;
+}
+
+template <int K>
+struct CostBytes {
+    uint32_t w[K / 4];
+};
+
+template <int K>
+__device__ __forceinline__ CostBytes<K> load_cost(const uint8_t* p) {
+    CostBytes<K> c;
+    if constexpr (K % 16 == 0) {
+#pragma unroll
+        for (int i = 0; i < K / 16; ++i) {
+            const u32x4 v = reinterpret_cast<const u32x4*>(p)[i];
+            c.w[4 * i + 0] = v.x;
+            c.w[4 * i + 1] = v.y;
+            c.w[4 * i + 2] = v.z;
+            c.w[4 * i + 3] = v.w;
+        }
+    } else if constexpr (K % 8 == 0) {
+#pragma unroll
+        for (int i = 0; i < K / 8; ++i) {
+            const u32x2 v = reinterpret_cast<const u32x2*>(p)[i];
+            c.w[2 * i + 0] = v.x;
+            c.w[2 * i + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < K / 4; ++i) c.w[i] = reinterpret_cast<const uint32_t*>(p)[i];
+    }
+    return c;
+}
+
+template <int K>
+__device__ __forceinline__ void store_path(uint16_t* p, const u16x2 (&v)[K / 2]) {
+    if constexpr (K % 8 == 0) {
+#pragma unroll
+        for (int i = 0; i < K / 8; ++i)
+            __builtin_nontemporal_store(
+                u32x4{bits(v[4 * i]), bits(v[4 * i + 1]), bits(v[4 * i + 2]), bits(v[4 * i + 3])},
+                reinterpret_cast<u32x4*>(p) + i);
+    } else {
+#pragma unroll
+        for (int i = 0; i < K / 4; ++i)
+            __builtin_nontemporal_store(u32x2{bits(v[2 * i]), bits(v[2 * i + 1])},
+                                        reinterpret_cast<u32x2*>(p) + i);
+    }
+}
+
+// uint8 costs -> packed t16(C * costScale).
+template <int K>
+__device__ __forceinline__ void expand_cost(const CostBytes<K>& c, u16x2 cs, u16x2 (&out)[K / 2]) {
+#pragma unroll
+    for (int i = 0; i < K / 4; ++i) {
+        out[2 * i] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c010c00u)) * cs;
+        out[2 * i + 1] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c030c02u)) * cs;
+    }
+}
+
+// d-1 and d+1 neighbour vectors of a packed per-lane slice (lane l holds d = l*K .. l*K+K-1).
+template <int K>
+__device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&dm1)[K / 2],
+                                           uint32_t (&dp1)[K / 2]) {
+    const uint32_t fromPrev = dpp<kRowShr1>(bits(v[K / 2 - 1]), 0xFFFFFFFFu);
+    const uint32_t fromNext = dpp<kRowShl1>(bits(v[0]), 0xFFFFFFFFu);
+#pragma unroll
+    for (int j = 0; j < K / 2; ++j) {
+        const uint32_t cur = bits(v[j]);
+        const uint32_t lo = j == 0 ? fromPrev : bits(v[j - 1]);
+        const uint32_t hi = j == K / 2 - 1 ? fromNext : bits(v[j + 1]);
+        dm1[j] = (cur << 16) | (lo >> 16);
+        dp1[j] = (cur >> 16) | (hi << 16);
+    }
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t slice_min(const u16x2 (&v)[K / 2]) {
+    u16x2 mm = v[0];
+#pragma unroll
+    for (int j = 1; j < K / 2; ++j) mm = vmin(mm, v[j]);
+    const uint32_t b = bits(mm);
+    return min(b & 0xFFFFu, b >> 16);
+}
+
+// The 256-entry t16(P2adj) table (src/sgbm_kernels.cu:304-310) lives in registers: lane l
+// computes entries 4l..4l+3 once per wave into two packed words; a lookup fetches the word
+// from lane g/4 with ds_bpermute.
+struct P2Table {
+    uint32_t lo, hi;
+};
+
+__device__ __forceinline__ P2Table make_p2(int P1, int P2, int lane) {
+    uint32_t e[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int g = 4 * lane + k;
+        const int v = max(P1 + 1, P2 * 8 / max(8, g));
+        e[k] = static_cast<uint32_t>(v) & 0xFFFFu;
+    }
+    return P2Table{e[0] | (e[1] << 16), e[2] | (e[3] << 16)};
+}
+
+__device__ __forceinline__ uint32_t p2_lookup(const P2Table& t, uint32_t g) {
+    const int src = static_cast<int>(g >> 2) * 4;  // byte address of the source lane
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(t.lo)));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(t.hi)));
+    const uint32_t w = (g & 2u) ? hi : lo;
+    return (g & 1u) ? (w >> 16) : (w & 0xFFFFu);
+}
+
+// Scanlines s0 .. s0+3 of `path` of pair j; wave-wide (no block synchronisation). Lanes of a
+// DPP row whose scanline does not exist compute on a copy of pixel 0 and store to `trash`.
+template <int K, bool PAD>
+__device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_base) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, l = lane & 15;
+    const int s = s0 + g;
+    int p0, step, len;
+    scan_geom(path, s, j.W, j.H, p0, step, len);
+    int maxLen = max(len, __shfl_xor(len, 16));
+    maxLen = max(maxLen, __shfl_xor(maxLen, 32));
+    maxLen = __builtin_amdgcn_readfirstlane(maxLen);
+    const P2Table p2t = make_p2(j.P1, j.P2, lane);
+
+    const int Dp = j.Dp, D = j.D;
+    const uint8_t* cbase = j.cost + l * K;
+    uint16_t* Lbase = j.L + (size_t)path * j.pathStride + l * K;
+    const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
+    const u16x2 P1p = splat(static_cast<uint32_t>(j.P1));
+    const uint32_t m0 = l == 0 ? 0xFFFFu : 0u;  // slot d = 0 has no d-1 neighbour
+    uint32_t c2m[K / 2], padm[K / 2];
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+        const int dlo = l * K + 2 * q;
+        c2m[q] = (dlo == D - 1 ? 0xFFFFu : 0u) | (dlo + 1 == D - 1 ? 0xFFFF0000u : 0u);
+        padm[q] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
+    }
+
+    // Origin: L = t16(C * costScale).
+    const long long cstep = (long long)step * Dp;  // bytes between consecutive cost vectors
+    const uint8_t* lp = cbase + (size_t)p0 * Dp;   // cost pointer of the last issued load
+    int lt = 0;                                     // step index of the last issued load
+    uint16_t* sp = Lbase + (size_t)p0 * Dp;         // path-cost pointer of the current step
+    uint16_t* const trash = trash_base + lane * K;  // stores of finished rows land here
+    u16x2 Lv[K / 2];
+    {
+        const CostBytes<K> c0 = load_cost<K>(lp);
+        expand_cost<K>(c0, cs, Lv);
+        if constexpr (PAD) {
+#pragma unroll
+            for (int q = 0; q < K / 2; ++q) Lv[q] = as16(bits(Lv[q]) | padm[q]);
+        }
+        store_path<K>(len > 0 ? sp : trash, Lv);
+    }
+    uint32_t M = row_min(slice_min<K>(Lv));
+
+    // Two register blocks of PF steps alternate: while one block is computed, the costs of the
+    // next are in flight. Lane l of a row also fetches the two image bytes of step
+    // tb + (l % PF) and looks up t16(P2adj) once per block; each step broadcasts its lane's
+    // value with DPP row_newbcast. Loads past the end of a scanline re-read its last pixel;
+    // steps past the end are computed but stored to the trash slot, so every store is
+    // unconditional and the wait counts stay exact.
+    constexpr int PF = K <= 8 ? 8 : (K <= 16 ? 4 : 2);
+    const int last = len > 0 ? len - 1 : 0;
+    const int lu = l & (PF - 1);
+    auto fetch = [&](int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
+        const int ti = min(tb + lu, last);
+        iv[0] = j.left[p0 + ti * step];
+        iv[1] = j.left[p0 + max(ti - 1, 0) * step];
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const bool adv = lt < last;
+            lp = adv ? lp + cstep : lp;
+            lt += adv ? 1 : 0;
+            dst[u] = load_cost<K>(lp);
+        }
+    };
+    auto run = [&](int tb, const CostBytes<K>(&blk)[PF], const int (&iv)[2]) {
+        const uint32_t p2v = p2_lookup(p2t, static_cast<uint32_t>(abs(iv[0] - iv[1])));
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int t = tb + u;
+            const bool act = t < len;
+            sp += step * Dp;
+            const u16x2 c3 = splat(M + row_bcast(p2v, u));
+            const u16x2 Mp = splat(M);
+            u16x2 Ch[K / 2];
+            expand_cost<K>(blk[u], cs, Ch);
+            uint32_t dm1[K / 2], dp1[K / 2];
+            neighbours<K>(Lv, dm1, dp1);
+#pragma unroll
+            for (int q = 0; q < K / 2; ++q) {
+                uint32_t c1 = bits(as16(dm1[q]) + P1p);
+                uint32_t c2 = bits(as16(dp1[q]) + P1p);
+                if (q == 0) c1 |= m0;
+                if constexpr (PAD) {
+                    c2 |= c2m[q];
+                } else {
+                    if (q == K / 2 - 1) c2 |= c2m[q];
+                }
+                const u16x2 mn = vmin(vmin(Lv[q], as16(c1)), vmin(as16(c2), c3));
+                u16x2 nv = Ch[q] + mn - Mp;
+                if constexpr (PAD) nv = as16(bits(nv) | padm[q]);
+                Lv[q] = nv;
+            }
+            store_path<K>(act ? sp : trash, Lv);
+            M = row_min(slice_min<K>(Lv));
+        }
+    };
+    CostBytes<K> bufA[PF], bufB[PF];
+    int gA[2], gB[2];
+    fetch(1, bufA, gA);
+    // Entering the loop with block A complete keeps the loop-header state identical on both
+    // edges, so no full drain is needed there.
+    LUMEN_WAIT_VMCNT(0);
+    for (int t0 = 1; t0 < maxLen; t0 += 2 * PF) {
+        // The scheduling barriers keep each block's loads where they are issued: moved next to
+        // their first use they would stall there.
+        fetch(t0 + PF, bufB, gB);
+        __builtin_amdgcn_sched_barrier(0);
+        run(t0, bufA, gA);
+        fetch(t0 + 2 * PF, bufA, gA);
+        __builtin_amdgcn_sched_barrier(0);
+        run(t0 + PF, bufB, gB);
+        // Block A's loads were issued before block B's stores: waiting until only those
+        // stores may be outstanding retires the loads without waiting on the stores.
+        constexpr int N = PF * (K % 8 == 0 ? K / 8 : K / 4);
+        static_assert(N < 64, "vmcnt is 6 bits");
+        LUMEN_WAIT_VMCNT(N);
+    }
+}
+
+// Issue priority of a wave by the length of its longest scanline: a launch ends when its
+// longest scanlines end, so those waves go first within their SIMD.
+__device__ __forceinline__ void set_priority_for_length(int len) {
+    if (len >= 1536)
+        __builtin_amdgcn_s_setprio(3);
+    else if (len >= 768)
+        __builtin_amdgcn_s_setprio(2);
+    else if (len >= 256)
+        __builtin_amdgcn_s_setprio(1);
+    else
+        __builtin_amdgcn_s_setprio(0);
+}
+
+// ---------------------------------------------------------------------------------------
+// Path sum + left WTA + right WTA + LR check of one image row, block-wide.
+//
+// Each DPP row of the block takes one pixel at a time: it sums the path costs (mod 2^16),
+// finds the left winner with a 4-step butterfly over (cost<<16 | d) keys plus the second
+// minimum (src/sgbm_kernels.cu:748-805), and offers every (pixel, d) to right pixel
+// x - m - d as the 64-bit key cost<<48 | d<<32 | S(d-1)<<16 | S(d+1) through an LDS atomic
+// min. Smallest key = smallest cost, then smallest d, which is the first-minimum rule of the
+// right WTA (src/sgbm_kernels.cu:820-834); costs equal to 65535 are never offered because
+// they never beat the reference's initial value. The LR check (src/sgbm_kernels.cu:862-890)
+// only reads the same row, so it runs here too.
+// ---------------------------------------------------------------------------------------
+template <int K>
+__host__ __device__ inline size_t wta_lds_bytes(int W) {
+    return (size_t)(W + W / K + 1) * 8 + (size_t)W * 4;
+}
+
+template <int K>
+__device__ __forceinline__ uint32_t pick32(const uint32_t (&v)[K / 2], int slot) {
+    uint32_t r = 0xFFFFu;
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+        r = slot == 2 * q ? (v[q] & 0xFFFFu) : r;
+        r = slot == 2 * q + 1 ? (v[q] >> 16) : r;
+    }
+    return r;
+}
+
+template <int K, bool PAD>
+__device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
+    // rbest[xr + xr/K]: running right-WTA key of right pixel xr. One padding entry every K
+    // entries spreads the 16 lanes of a row (whose candidates are K right pixels apart) over
+    // distinct LDS banks.
+    const int W = j.W, D = j.D, Dp = j.Dp, m = j.m;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int nslots = W + W / K + 1;
+    int16_t* sdl = reinterpret_cast<int16_t*>(rbest + nslots);
+    int16_t* sdr = sdl + W;
+    for (int i = tid; i < nslots; i += nt) rbest[i] = ~0ull;
+    __syncthreads();
+    const int g = tid >> 4, l = tid & 15, ngroups = nt >> 4;
+    uint32_t padm[K / 2];
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+        const int dlo = l * K + 2 * q;
+        padm[q] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
+    }
+    const size_t rowPix = (size_t)y * W;
+    for (int x = g; x < W; x += ngroups) {
+        const uint16_t* src = j.L + (rowPix + x) * Dp + l * K;
+        u16x2 S[K / 2];
+#pragma unroll
+        for (int q = 0; q < K / 2; ++q) S[q] = as16(0u);
+        for (int r = 0; r < j.npaths; ++r) {
+            const uint16_t* p = src + (size_t)r * j.pathStride;
+            if constexpr (K % 8 == 0) {
+#pragma unroll
+                for (int i = 0; i < K / 8; ++i) {
+                    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p) + i);
+                    S[4 * i + 0] += as16(v.x);
+                    S[4 * i + 1] += as16(v.y);
+                    S[4 * i + 2] += as16(v.z);
+                    S[4 * i + 3] += as16(v.w);
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < K / 4; ++i) {
+                    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p) + i);
+                    S[2 * i + 0] += as16(v.x);
+                    S[2 * i + 1] += as16(v.y);
+                }
+            }
+        }
+        if constexpr (PAD) {
+#pragma unroll
+            for (int q = 0; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
+        }
+        uint32_t sm1[K / 2], sp1[K / 2];
+        neighbours<K>(S, sm1, sp1);
+
+        // Left WTA: (first minimum, second minimum of the multiset).
+        uint32_t k1 = 0xFFFFFFFFu, m2 = 0xFFFFu;
+#pragma unroll
+        for (int q = 0; q < K / 2; ++q) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
+                const uint32_t key = (v << 16) | static_cast<uint32_t>(l * K + 2 * q + h);
+                m2 = min(m2, max(k1, key) >> 16);
+                k1 = min(k1, key);
+            }
+        }
+#define LUMEN_WTA_COMBINE(CTRL)                   \
+    {                                             \
+        const uint32_t ok = dpp<CTRL>(k1, k1);    \
+        const uint32_t om = dpp<CTRL>(m2, m2);    \
+        m2 = min(min(m2, om), max(k1, ok) >> 16); \
+        k1 = min(k1, ok);                         \
+    }
+        LUMEN_WTA_COMBINE(kQuadXor1)
+        LUMEN_WTA_COMBINE(kQuadXor2)
+        LUMEN_WTA_COMBINE(kRowHalfMirror)
+        LUMEN_WTA_COMBINE(kRowMirror)
+#undef LUMEN_WTA_COMBINE
+        const int bestD = static_cast<int>(k1 & 0xFFFFu);
+        const uint32_t best = k1 >> 16;
+        if (l == bestD / K) {
+            int16_t out;
+            const uint32_t ratio = static_cast<uint32_t>(j.uniquenessRatio);
+            if (j.uniquenessRatio > 0 && m2 != 0xFFFFu && m2 <= best + (best * ratio) / 100u) {
+                out = -1;
+            } else {
+                float sub = static_cast<float>(bestD);
+                if (bestD > 0 && bestD < D - 1) {
+                    const int slot = bestD - l * K;
+                    sub = parabola(sub, pick32<K>(sm1, slot), best, pick32<K>(sp1, slot));
+                }
+                out = f2i16((sub + static_cast<float>(m)) * 16.0f);
+            }
+            j.dispL[rowPix + x] = out;
+            j.wtaInt[rowPix + x] = static_cast<int16_t>(bestD);
+            sdl[x] = out;
+        }
+
+        // Right WTA candidates.
+#pragma unroll
+        for (int q = 0; q < K / 2; ++q) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int d = l * K + 2 * q + h;
+                const int xr = x - m - d;
+                const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
+                if (d < D && xr >= 0 && v != 0xFFFFu) {
+                    const uint32_t lo = h ? (sm1[q] >> 16) : (sm1[q] & 0xFFFFu);
+                    const uint32_t hi = h ? (sp1[q] >> 16) : (sp1[q] & 0xFFFFu);
+                    const unsigned long long key = (static_cast<unsigned long long>(v) << 48) |
+                                                   (static_cast<unsigned long long>(d) << 32) |
+                                                   (lo << 16) | hi;
+                    atomicMin(&rbest[xr + xr / K], key);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int xr = tid; xr < W; xr += nt) {
+        const unsigned long long key = rbest[xr + xr / K];
+        float sub = 0.0f;
+        if (key != ~0ull) {
+            const int bd = static_cast<int>((key >> 32) & 0xFFFFu);
+            sub = static_cast<float>(bd);
+            if (bd > 0 && bd < D - 1)
+                sub = parabola(sub, static_cast<uint32_t>((key >> 16) & 0xFFFFu),
+                               static_cast<uint32_t>(key >> 48), static_cast<uint32_t>(key & 0xFFFFu));
+        }
+        const int16_t dr = f2i16((sub + static_cast<float>(m)) * 16.0f);
+        j.dispR[rowPix + xr] = dr;
+        sdr[xr] = dr;
+    }
+    __syncthreads();
+    for (int x = tid; x < W; x += nt) {
+        int16_t dl = sdl[x];
+        if (j.doLR && dl > 0) {
+            const int xr = x - dl / 16;
+            if (xr < 0 || xr >= W) {
+                dl = -1;
+            } else {
+                const int16_t dr = sdr[xr];
+                if (dr <= 0 || abs(int(dl) - int(dr)) > j.lrMaxDiff) dl = -1;
+            }
+        }
+        j.dispLR[rowPix + x] = dl;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Speckle filter (src/sgbm_kernels.cu:979-1013) with snapshot semantics: reads `in`, writes
+// `out`, so the result does not depend on thread timing. One pixel.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void speckle_px(const int16_t* __restrict__ in, int16_t* __restrict__ out,
+                                           int W, int H, int x, int y, int minSimilar,
+                                           int maxDiff) {
+    const int16_t c = in[(size_t)y * W + x];
+    int16_t res = c;
+    if (c > 0) {
+        int n = 0;
+        for (int dy = -2; dy <= 2; ++dy)
+            for (int dx = -2; dx <= 2; ++dx) {
+                const int nx = x + dx, ny = y + dy;
+                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
+                const int16_t v = in[(size_t)ny * W + nx];
+                if (v > 0 && abs(int(c) - int(v)) <= maxDiff) ++n;
+            }
+        if (n < minSimilar) res = -1;
+    }
+    out[(size_t)y * W + x] = res;
+}
+
+// ---------------------------------------------------------------------------------------
+// 3x3 median of positive values (src/sgbm_kernels.cu:892-935) over one 64x4 tile,
+// block-wide (256 threads). The tile and a 1-pixel halo are staged in LDS; out-of-image taps
+// are staged as 0, which the positive-only rule excludes. The nine taps (non-positive ones
+// replaced by INT_MAX so they sort last) go through a fixed 25-comparator sorting network;
+// the upper median is element n/2 of the sorted positives.
+// ---------------------------------------------------------------------------------------
+constexpr int kMedTW = 64, kMedTH = 4;
+
+__device__ __forceinline__ void cswap(int& a, int& b) {
+    const int lo = min(a, b);
+    b = max(a, b);
+    a = lo;
+}
+
+__device__ void median_tile(const int16_t* __restrict__ in, int16_t* __restrict__ out, int W,
+                            int H, int x0, int y0, int16_t (*tile)[kMedTW + 2]) {
+    for (int i = threadIdx.x; i < (kMedTH + 2) * (kMedTW + 2); i += blockDim.x) {
+        const int ty = i / (kMedTW + 2), tx = i % (kMedTW + 2);
+        const int gx = x0 + tx - 1, gy = y0 + ty - 1;
+        tile[ty][tx] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? in[(size_t)gy * W + gx] : int16_t(0);
+    }
+    __syncthreads();
+    const int tx = threadIdx.x % kMedTW, ty = threadIdx.x / kMedTW;
+    const int x = x0 + tx, y = y0 + ty;
+    if (ty < kMedTH && x < W && y < H) {
+        int v[9];
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int t = tile[ty + k / 3][tx + k % 3];
+            n += t > 0 ? 1 : 0;
+            v[k] = t > 0 ? t : 0x7FFFFFFF;
+        }
+        cswap(v[0], v[1]); cswap(v[3], v[4]); cswap(v[6], v[7]);
+        cswap(v[1], v[2]); cswap(v[4], v[5]); cswap(v[7], v[8]);
+        cswap(v[0], v[1]); cswap(v[3], v[4]); cswap(v[6], v[7]);
+        cswap(v[0], v[3]); cswap(v[3], v[6]); cswap(v[0], v[3]);
+        cswap(v[1], v[4]); cswap(v[4], v[7]); cswap(v[1], v[4]);
+        cswap(v[2], v[5]); cswap(v[5], v[8]); cswap(v[2], v[5]);
+        cswap(v[1], v[3]); cswap(v[5], v[7]); cswap(v[2], v[6]);
+        cswap(v[4], v[6]); cswap(v[2], v[4]); cswap(v[2], v[3]);
+        cswap(v[5], v[6]);
+        const int k = n >> 1;
+        int r = v[0];
+#pragma unroll
+        for (int q = 1; q < 9; ++q) r = k == q ? v[q] : r;
+        out[(size_t)y * W + x] = n == 0 ? tile[ty + 1][tx + 1] : static_cast<int16_t>(r);
+    }
+}
+
+}  // namespace lumen

@@ -1,0 +1,69 @@
+// sgbm_host.h - host helpers shared by the per-pair, batched and serving executors.
+#pragma once
+#include <algorithm>
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "lumen_sgbm.h"
+#include "sgbm_internal.h"
+
+namespace lumen {
+
+// 4 paths, or 8 when numDirections >= 8 (src/sgbm_gpu.cu:321-421).
+inline int num_paths(const lumen_stereo_params& p) { return p.numDirections >= 8 ? 8 : 4; }
+
+// src/sgbm_gpu.cu:312-317: SAD uses blockSize^2, every other mode the census formula.
+inline int cost_scale(const lumen_stereo_params& p) {
+    if (p.matchingCostMode == 0) return p.blockSize * p.blockSize;
+    return std::max(1, (p.blockSize * p.blockSize * 2 * p.preFilterCap) / 255);
+}
+
+// Scalar fields of a PairJob from the parameters and image size (pointers untouched).
+inline void fill_job(PairJob& j, const lumen_stereo_params& p, int W, int H) {
+    j.W = W;
+    j.H = H;
+    j.m = p.minDisparity;
+    j.D = p.maxDisparity - p.minDisparity;
+    j.Dp = padded_range(j.D);
+    j.K = j.Dp / kScanLanes;
+    j.pathStride = (size_t)W * H * j.Dp;
+    j.mode = p.matchingCostMode;
+    j.blockSize = p.blockSize;
+    j.preFilterCap = p.preFilterCap;
+    j.P1 = p.P1;
+    j.P2 = p.P2;
+    j.costScale = cost_scale(p);
+    j.npaths = num_paths(p);
+    j.uniquenessRatio = p.uniquenessRatio;
+    j.doLR = p.disp12MaxDiff >= 0 ? 1 : 0;
+    j.lrMaxDiff = p.disp12MaxDiff * 16;
+    j.doSpeckle = p.speckleWindowSize > 0 ? 1 : 0;
+    j.speckleMin = std::min(p.speckleWindowSize, 25);
+    j.speckleDiff = p.speckleRange * 16;
+}
+
+// Device bytes of one pair's working set: images, padded cost volume, per-path uint16
+// buffers and the six int16 maps.
+inline size_t pair_footprint(const lumen_stereo_params& p, int W, int H) {
+    const size_t n = (size_t)W * H;
+    const size_t vol = n * padded_range(p.maxDisparity - p.minDisparity);
+    return vol + vol * 2 * num_paths(p) + n * 2 + n * 2 * 6;
+}
+
+// Validation of src/sgbm_gpu.cu:148-166 plus the limits of this implementation; returns ""
+// when the pair is accepted, otherwise the message the reference (or this build) gives.
+inline std::string validate_pair(const lumen_stereo_params& p, int W, int H, bool haveImages) {
+    if (!haveImages || W <= 0 || H <= 0) return "Input images are empty";
+    const int D = p.maxDisparity - p.minDisparity;
+    if (D < 1) return "Invalid disparity range: maxDisparity must be greater than minDisparity";
+    if (D > LUMEN_MAX_DISPARITY_RANGE)
+        return "Disparity range (maxDisparity - minDisparity) exceeds GPU limit (" +
+               std::to_string(LUMEN_MAX_DISPARITY_RANGE) + "). Reduce the search range.";
+    if (p.minDisparity < 0) return "minDisparity must be >= 0";
+    if (p.matchingCostMode != 1 && W < 2) return "Image width must be at least 2 for the SAD prefilter";
+    if (W > kMaxWidth) return "Image width exceeds " + std::to_string(kMaxWidth);
+    return "";
+}
+
+}  // namespace lumen
