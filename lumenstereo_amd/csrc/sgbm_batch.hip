@@ -361,7 +361,7 @@ void build_plan(Plan& pl) {
         if (j.mode == 1) {
             cost = ((j.W + kCensusTile - 1) / kCensusTile) * j.H;
             pl.costLds = std::max(pl.costLds, census_lds_bytes(j.Dp));
-        } else if (r >= 0 && r <= 31) {
+        } else if (r >= 0 && sad_lds_bytes(r, j.Dp) <= kSadMaxLds) {
             cost = ((j.W + kSadTile - 1) / kSadTile) * j.H;
             pl.costLds = std::max(pl.costLds, sad_lds_bytes(r, j.Dp));
         } else {

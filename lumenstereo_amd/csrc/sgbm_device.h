@@ -122,9 +122,14 @@ __host__ __device__ inline size_t census_lds_bytes(int Dp) {
     return (size_t)(kCensusTile + nr) * 4 + 5 * (kCensusTile + 4) + 5 * (nr + 4);
 }
 
+// SAD segment LDS: prefiltered left/right tiles stored column-major (the 2r+1 rows of a column
+// packed four per dword), then the segment's output tile [kSadTile][Dp].
 __host__ __device__ inline size_t sad_lds_bytes(int r, int Dp) {
-    return (size_t)(2 * r + 1) * ((kSadTile + 2 * r) + (kSadTile + 2 * r + Dp - 1));
+    const int nq = (2 * r + 1 + 3) / 4;
+    return (size_t)(kSadTile + 2 * r) * nq * 4 + (size_t)(kSadTile + 2 * r + Dp - 1) * nq * 4 +
+           (size_t)kSadTile * Dp;
 }
+constexpr size_t kSadMaxLds = 64 * 1024;
 
 __device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, int c) {
     // rows: 5 staged image rows, column c is the centre's column in the staging buffer.
@@ -254,75 +259,139 @@ __device__ __forceinline__ uint8_t prefilter_at(const uint8_t* __restrict__ row,
     return static_cast<uint8_t>(g + cap);
 }
 
-// Block-wide; every thread of the block must call it with the same (y, x0). Needs r <= 31.
-__device__ void sad_cost_segment(const PairJob& j, int y, int x0, uint8_t* shs) {
+// Block-wide; every thread of the block must call it with the same (y, x0); the segment's LDS
+// (sad_lds_bytes) must fit kSadMaxLds.
+//
+// For disparity d the window sum is a sliding sum over columns: col(c) is the SAD of the
+// 2r+1 staged rows of left column c against right column c - (m+d), two v_sad_u8 per column
+// for r <= 3, and zero when either column is outside the image (its taps do not count). A
+// thread owns one disparity and one part of the segment and slides along x; R > 0 keeps the
+// last 2R+1 columns in registers, R = 0 (any radius) recomputes the column that leaves.
+template <int R>
+__device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs) {
     const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
-    const int r = j.blockSize / 2;
+    const int r = R > 0 ? R : j.blockSize / 2;
     const int cap = j.preFilterCap > 1 ? j.preFilterCap : 1;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int lw = kSadTile + 2 * r;
-    const int rw = kSadTile + 2 * r + Dp - 1;
-    const int rows = 2 * r + 1;
-    uint8_t* fl = shs;
-    uint8_t* fr = shs + rows * lw;
-    const int ls = x0 - r, rs = x0 - r - m - (Dp - 1);
-    for (int k = 0; k < rows; ++k) {
-        const int yy = y - r + k;
-        const bool rowIn = yy >= 0 && yy < H;
-        for (int i = tid; i < lw; i += nt) {
-            const int c = ls + i;
-            fl[k * lw + i] = (rowIn && c >= 0 && c < W)
-                                 ? prefilter_at(j.left + (size_t)yy * W, c, W, cap) : uint8_t(0);
-        }
-        for (int i = tid; i < rw; i += nt) {
-            const int c = rs + i;
-            fr[k * rw + i] = (rowIn && c >= 0 && c < W)
-                                 ? prefilter_at(j.right + (size_t)yy * W, c, W, cap) : uint8_t(0);
-        }
+    const int rows = 2 * r + 1, nq = (rows + 3) / 4;
+    const int TL = kSadTile + 2 * r, TR = kSadTile + 2 * r + Dp - 1;
+    uint32_t* lt = reinterpret_cast<uint32_t*>(shs);  // TL columns x nq dwords
+    uint32_t* rt = lt + TL * nq;                          // TR columns x nq dwords
+    uint8_t* ot = reinterpret_cast<uint8_t*>(rt + TR * nq);  // [kSadTile][Dp]
+    const int lbase = x0 - r, rbase = x0 - r - m - (Dp - 1);
+    uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
+    uint8_t* rb = reinterpret_cast<uint8_t*>(rt);
+    for (int i = tid; i < TL * nq * 4; i += nt) {
+        const int c = i / (nq * 4), k = i - c * (nq * 4);
+        const int gc = lbase + c, yy = y - r + k;
+        lb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
+                    ? prefilter_at(j.left + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+    }
+    for (int i = tid; i < TR * nq * 4; i += nt) {
+        const int c = i / (nq * 4), k = i - c * (nq * 4);
+        const int gc = rbase + c, yy = y - r + k;
+        rb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
+                    ? prefilter_at(j.right + (size_t)yy * W, gc, W, cap) : uint8_t(0);
     }
     __syncthreads();
     const int dylo = max(-r, -y), dyhi = min(r, H - 1 - y);
     const int nrows = dyhi - dylo + 1;
-    const int chunks = Dp / 16;
-    for (int item = tid; item < kSadTile * chunks; item += nt) {
-        const int i = item / chunks, c = item - i * chunks;
-        const int x = x0 + i;
-        if (x >= W) continue;
-        const int dxhi = min(r, W - 1 - x);
-        const int d0 = c * 16;
-        uint32_t w[4];
+    const uint32_t full = static_cast<uint32_t>(rows * rows);
+    // floor(s / full) == umulhi(s, ceil(2^32 / full)) for s <= 255 * full (255 * full^2 < 2^32).
+    const uint32_t magic = static_cast<uint32_t>((0x100000000ull + full - 1) / full);
+    const int parts = max(1, nt / Dp);
+    const int plen = (kSadTile + parts - 1) / parts;
+    for (int t = tid; t < Dp * parts; t += nt) {
+        const int d = t % Dp, part = t / Dp;
+        const int i0 = part * plen, i1 = min(kSadTile, i0 + plen);
+        if (i0 >= i1) continue;
+        if (d >= D) {
+            for (int i = i0; i < i1; ++i) ot[i * Dp + d] = 255;
+            continue;
+        }
+        const int a = m + d;
+        // col(c): left tile column c (image column lbase + c) against right tile column
+        // c + Dp - 1 - d (image column lbase + c - a).
+        auto col = [&](int c) -> uint32_t {
+            const int gc = lbase + c;
+            if (gc < a || gc >= W) return 0u;
+            const uint32_t* L = lt + c * nq;
+            const uint32_t* Rr = rt + (c + Dp - 1 - d) * nq;
+            uint32_t acc = 0;
+            if constexpr (R > 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint32_t word = 0;
+                for (int q = 0; q < (2 * R + 1 + 3) / 4; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
+            } else {
+                for (int q = 0; q < nq; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
+            }
+            return acc;
+        };
+        auto emit = [&](int i, uint32_t sum) {
+            const int x = x0 + i;
+            const int dxlo = max(-r, a - x), dxhi = min(r, W - 1 - x);
+            const int ncols = dxhi - dxlo + 1;
+            uint32_t v;
+            if (ncols == rows && nrows == rows)
+                v = __umulhi(sum, magic);
+            else if (ncols > 0 && nrows > 0)
+                v = sum / static_cast<uint32_t>(ncols * nrows);
+            else
+                v = 255;
+            ot[i * Dp + d] = static_cast<uint8_t>(v);
+        };
+        // Window of output pixel i covers tile columns i .. i + 2r.
+        if constexpr (R > 0) {
+            constexpr int NW = 2 * R + 1;
+            uint32_t ring[NW];
+            uint32_t sum = 0;
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int d = d0 + q * 4 + b;
-                uint32_t v = 255;
-                if (d < D) {
-                    const int a = m + d;
-                    const int dxlo = max(-r, a - x);
-                    const int ncols = dxhi - dxlo + 1;
-                    if (ncols > 0 && nrows > 0) {
-                        int sum = 0;
-                        for (int dy = dylo; dy <= dyhi; ++dy) {
-                            const uint8_t* lrow = fl + (r + dy) * lw + i + r;
-                            const uint8_t* rrow = fr + (r + dy) * rw + i + r + Dp - 1 - d;
-                            for (int dx = dxlo; dx <= dxhi; ++dx)
-                                sum += abs(int(lrow[dx]) - int(rrow[dx]));
-                        }
-                        v = static_cast<uint32_t>(sum / (ncols * nrows));
+            for (int k = 0; k < NW - 1; ++k) {
+                ring[k] = col(i0 + k);
+                sum += ring[k];
+            }
+            ring[NW - 1] = 0;
+            for (int i = i0; i < i1; i += NW) {
+#pragma unroll
+                for (int u = 0; u < NW; ++u) {
+                    if (i + u < i1) {
+                        const int slot = (u + NW - 1) % NW;  // column i + u + 2R enters here
+                        ring[slot] = col(i + u + 2 * R);
+                        sum += ring[slot];
+                        emit(i + u, sum);
+                        sum -= ring[u];                       // column i + u leaves
                     }
                 }
-                word |= (v & 0xFFu) << (8 * b);
             }
-            w[q] = word;
+        } else {
+            uint32_t sum = 0;
+            for (int k = 0; k < 2 * r; ++k) sum += col(i0 + k);
+            for (int i = i0; i < i1; ++i) {
+                sum += col(i + 2 * r);
+                emit(i, sum);
+                sum -= col(i);
+            }
         }
-        *reinterpret_cast<u32x4*>(j.cost + ((size_t)y * W + x) * Dp + d0) =
-            u32x4{w[0], w[1], w[2], w[3]};
+    }
+    __syncthreads();
+    // The segment's costs are contiguous in the volume: 16-byte stores.
+    const int npix = min(kSadTile, W - x0);
+    const int n16 = npix * Dp / 16;
+    u32x4* dst = reinterpret_cast<u32x4*>(j.cost + ((size_t)y * W + x0) * Dp);
+    const u32x4* src = reinterpret_cast<const u32x4*>(ot);
+    for (int i = tid; i < n16; i += nt) dst[i] = src[i];
+}
+
+__device__ void sad_cost_segment(const PairJob& j, int y, int x0, uint8_t* shs) {
+    switch (j.blockSize / 2) {
+        case 1: sad_cost_segment_t<1>(j, y, x0, shs); break;
+        case 2: sad_cost_segment_t<2>(j, y, x0, shs); break;
+        case 3: sad_cost_segment_t<3>(j, y, x0, shs); break;
+        default: sad_cost_segment_t<0>(j, y, x0, shs); break;
     }
 }
 
-// SAD cost of one (pixel, disparity) read from the images directly (windows with r > 31).
+// SAD cost of one (pixel, disparity) read from the images directly (windows whose segment does
+// not fit in LDS).
 __device__ __forceinline__ uint8_t sad_cost_direct(const PairJob& j, int x, int y, int d) {
     const int W = j.W, H = j.H;
     const int r = j.blockSize / 2;
@@ -564,7 +633,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     // value with DPP row_newbcast. Loads past the end of a scanline re-read its last pixel;
     // steps past the end are computed but stored to the trash slot, so every store is
     // unconditional and the wait counts stay exact.
-    constexpr int PF = K <= 8 ? 8 : (K <= 16 ? 4 : 2);
+    constexpr int PF = K <= 8 ? 8 : 4;
     const int last = len > 0 ? len - 1 : 0;
     const int lu = l & (PF - 1);
     auto fetch = [&](int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {

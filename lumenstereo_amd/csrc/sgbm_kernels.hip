@@ -19,7 +19,8 @@ __global__ __launch_bounds__(256) void cost_sad_kernel(PairJob j) {
     sad_cost_segment(j, blockIdx.y, blockIdx.x * kSadTile, shs);
 }
 
-// Windows with radius > 31 read the images from global memory, one (pixel, d) per thread.
+// Windows too large for the LDS segment read the images from global memory, one (pixel, d)
+// per thread.
 __global__ __launch_bounds__(256) void cost_sad_direct_kernel(PairJob j) {
     const size_t item = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t total = (size_t)j.W * j.H * j.Dp;
@@ -40,7 +41,7 @@ hipError_t launch_cost(const PairJob& j, hipStream_t s) {
         // A negative radius gives an empty window: every cost is 255.
         return hipMemsetAsync(j.cost, 0xFF, (size_t)j.W * j.H * j.Dp, s);
     }
-    if (r <= 31) {
+    if (sad_lds_bytes(r, j.Dp) <= kSadMaxLds) {
         dim3 grid((j.W + kSadTile - 1) / kSadTile, j.H);
         hipLaunchKernelGGL(cost_sad_kernel, grid, dim3(256), sad_lds_bytes(r, j.Dp), s, j);
     } else {
