@@ -13,13 +13,24 @@ W*H*D * 3 * paths (read the uint8 cost once per path, write each uint16 path cos
 by its mean launch time from hipEvents recorded around it on the stream it runs on.
 cpu_baseline = the oracle's algorithm (oracle/sgbm_oracle.cpp, paths on 8 threads) on the same
 frame; OpenCV is not available on this image, so the OpenCV StereoSGBM baseline cannot run.
+
+Two more objects in the same line measure the other forms on the uneven 64-pair mix
+(lumenstereo_amd/workloads.py):
+  batched  - pairs/s and MDE/s of the batched executor against the per-pair loop on the same GPU,
+             with host-memory and with device-resident images, and how many of the 64 outputs
+             are bit-identical to the per-pair path's;
+  serving  - an open-loop Poisson sweep (20..400 pairs/s; the mix plus 10% 2880x1988 D=192
+             requests): p50/p99/p99.9 latency and sustained pairs/s of the server against the
+             per-pair path serving the same arrival trace in order, and bit-identity of every
+             served result.
 """
 from __future__ import annotations
 
 import argparse
+import gc
 import json
-import os
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -73,6 +84,215 @@ def cpu_baseline(ls, params, left, right, gpu_out, threads):
     }, mismatches
 
 
+def _pct(v, q):
+    return round(float(np.percentile(np.asarray(v, np.float64), q, method="inverted_cdf")), 3) if v else None
+
+
+def bench_batched(ls, torch, specs, imgs, plain_outs, reps):
+    """Per-pair loop vs batched executor over the whole mix, host and device images."""
+    n = len(specs)
+    total_de = sum(s.de for s in specs)
+    m = ls.StereoSGBM()
+
+    def plain_host():
+        for s, (l, r) in zip(specs, imgs):
+            m.setParams(s.params)
+            m.compute(l, r)
+
+    dev = torch.device("cuda:0")
+    d_in = [(torch.from_numpy(l).to(dev), torch.from_numpy(r).to(dev)) for l, r in imgs]
+    d_out = [torch.empty(l.shape, dtype=torch.int16, device=dev) for l, _ in imgs]
+
+    def plain_dev():
+        for s, (dl, dr), do in zip(specs, d_in, d_out):
+            m.setParams(s.params)
+            m.compute_device(dl.data_ptr(), dr.data_ptr(), s.width, s.height, do.data_ptr(), sync=False)
+        torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()  # warm-up: buffers, graphs and pinned staging reach their size
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    t_plain_host = timed(plain_host)
+    t_plain_dev = timed(plain_dev)
+    del m
+    gc.collect()
+    free, _ = torch.cuda.mem_get_info()
+    b = ls.BatchSGBM(arena_bytes=int(free * 0.8))
+    pairs = [(s.params, l, r) for s, (l, r) in zip(specs, imgs)]
+    batch_outs = b.run(pairs)
+    identical = sum(int(np.array_equal(a, o)) for a, o in zip(batch_outs, plain_outs))
+    t_batch_host = timed(lambda: b.run(pairs))
+    host_split = [round(float(x), 3) for x in b.timings()]
+    descs = (ls.CPairDesc * n)()
+    for i, (s, (dl, dr), do) in enumerate(zip(specs, d_in, d_out)):
+        descs[i] = ls.CPairDesc(s.params.to_c(), s.width, s.height, dl.data_ptr(), dr.data_ptr(),
+                                do.data_ptr())
+    t_batch_dev = timed(lambda: b.run_device(descs, n))
+    dev_identical = sum(int(np.array_equal(do.cpu().numpy(), o)) for do, o in zip(d_out, plain_outs))
+    arena = b.arena_bytes()
+    del b, d_in, d_out
+    gc.collect()
+    torch.cuda.empty_cache()
+
+    def rate(t):
+        return {"pairs_per_s": round(n / t, 1), "MDE_per_s": round(total_de / t / 1e6, 1),
+                "ms_per_batch": round(t * 1e3, 3)}
+
+    return {
+        "workload": f"{n} uneven pairs (W in {{320,640,1280,1920}}, H 9/16 or 3/4 W, D in "
+                    "{64,96,128,192}, SAD/Census, 4/8 paths), one batch",
+        "total_MDE": round(total_de / 1e6, 1),
+        "plain_loop_host_io": rate(t_plain_host),
+        "plain_loop_device_io": rate(t_plain_dev),
+        "batched_host_io": rate(t_batch_host),
+        "batched_device_io": rate(t_batch_dev),
+        "speedup_host_io": round(t_plain_host / t_batch_host, 3),
+        "speedup_device_io": round(t_plain_dev / t_batch_dev, 3),
+        "batched_host_split_ms": {"total": host_split[0], "upload": host_split[1],
+                                  "compute": host_split[2], "download": host_split[3]},
+        "bit_identical_pairs": f"{identical}/{n} host, {dev_identical}/{n} device",
+        "arena_bytes": arena,
+        "timing": f"best of {reps} after one warm-up",
+    }
+
+
+def _wait_until(t_ns, ls):
+    while True:
+        now = ls.now_ns()
+        if now >= t_ns:
+            return now
+        dt = (t_ns - now) / 1e9
+        time.sleep(dt - 0.0003 if dt > 0.001 else 0)
+
+
+def run_served(ls, srv, trace, entries):
+    """Submits trace[k] = (arrival s, entry) at its arrival time from a thread; returns the
+    latencies (ms, from the scheduled arrival), the outputs and the request infos."""
+    N = len(trace)
+    t0 = ls.now_ns() + 20_000_000
+    tickets = [0] * N
+    sem = threading.Semaphore(0)
+
+    def submitter():
+        for k, (off, e) in enumerate(trace):
+            _wait_until(t0 + int(off * 1e9), ls)
+            p, l, r = entries[e]
+            tickets[k] = srv.submit(p, l, r)
+            sem.release()
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    lats, outs, infos = [], [], []
+    for k in range(N):
+        sem.acquire()
+        out, info = srv.wait(tickets[k])
+        lats.append((info["t_done_ns"] - (t0 + int(trace[k][0] * 1e9))) / 1e6)
+        outs.append(out)
+        infos.append(info)
+    th.join()
+    span = (max(i["t_done_ns"] for i in infos) - t0) / 1e9
+    return lats, outs, infos, span
+
+
+def run_plain_fifo(ls, m, trace, entries):
+    """The per-pair path serving the same trace: one request at a time, in arrival order."""
+    t0 = ls.now_ns() + 20_000_000
+    lats = []
+    for off, e in trace:
+        arrival = t0 + int(off * 1e9)
+        if ls.now_ns() < arrival:
+            _wait_until(arrival, ls)
+        p, l, r = entries[e]
+        m.setParams(p)
+        m.compute(l, r)
+        lats.append((ls.now_ns() - arrival) / 1e6)
+    return lats, (ls.now_ns() - t0) / 1e9
+
+
+def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
+    from lumenstereo_amd import workloads
+
+    # Request pool: the 64 mix pairs plus four 2880x1988 D=192 pairs (both cost modes, 4/8 paths).
+    entries = [(s.params, l, r) for s, (l, r) in zip(specs, imgs)]
+    want = list(plain_outs)
+    m = ls.StereoSGBM()
+    big = []
+    for k, (census, paths) in enumerate([(True, 8), (False, 4), (True, 4), (False, 8)]):
+        W, H, D = workloads.BIG
+        spec = workloads.PairSpec(W, H, workloads.pair_params(D, census, paths), SEED + 5000 + k)
+        l, r = spec.images()
+        big.append(len(entries))
+        entries.append((spec.params, l, r))
+        m.setParams(spec.params)
+        want.append(m.compute(l, r))
+    # Alone latency of every entry through the per-pair path (host images, after a warm-up).
+    alone = []
+    for p, l, r in entries:
+        m.setParams(p)
+        t0 = time.perf_counter()
+        m.compute(l, r)
+        alone.append((time.perf_counter() - t0) * 1e3)
+    free, _ = torch.cuda.mem_get_info()
+    srv = ls.ServeSGBM(pool_bytes=int(free * 0.6), lanes=lanes, max_batch=64)
+    # Warm-up: every entry once, in one burst and alone, so graphs and staging reach their size.
+    for t in [srv.submit(*e) for e in entries]:
+        srv.wait(t)
+    rng = np.random.default_rng(SEED + 77)
+    sweep, mismatched, total = [], 0, 0
+    for rate in rates:
+        N = max(40, int(rate * seconds))
+        gaps = rng.exponential(1.0 / rate, N)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        pick = [int(rng.choice(big)) if rng.random() < 0.1 else int(rng.integers(0, len(specs)))
+                for _ in range(N)]
+        trace = list(zip(arrivals.tolist(), pick))
+        lats, outs, infos, span = run_served(ls, srv, trace, entries)
+        for (off, e), o in zip(trace, outs):
+            total += 1
+            mismatched += int(not np.array_equal(o, want[e]))
+        plat, pspan = run_plain_fifo(ls, m, trace, entries)
+        print(f"# rate {rate:g}/s: {N} requests, served p99 {_pct(lats, 99)} ms, plain p99 "
+              f"{_pct(plat, 99)} ms", file=sys.stderr, flush=True)
+        alone_tr = [alone[e] for _, e in trace]
+        sweep.append({
+            "rate": rate, "requests": N,
+            "served": {"p50_ms": _pct(lats, 50), "p99_ms": _pct(lats, 99),
+                       "p999_ms": _pct(lats, 99.9), "pairs_per_s": round(N / span, 1),
+                       "mean_batch": round(float(np.mean([i["batch_size"] for i in infos])), 2)},
+            "plain": {"p50_ms": _pct(plat, 50), "p99_ms": _pct(plat, 99),
+                      "p999_ms": _pct(plat, 99.9), "pairs_per_s": round(N / pspan, 1)},
+            "p99_served_over_plain": round(_pct(lats, 99) / _pct(plat, 99), 3),
+            "p99_served_over_alone_p99": round(_pct(lats, 99) / _pct(alone_tr, 99), 3),
+        })
+    st = srv.stats()
+    srv.close()
+    del srv, m
+    gc.collect()
+    return {
+        "workload": "open-loop Poisson arrivals; each request one of the 64 mix pairs, or with "
+                    "probability 0.1 one of four 2880x1988 D=192 pairs; latency from the "
+                    "scheduled arrival to the output in host memory",
+        "lanes": lanes,
+        "sweep": sweep,
+        "plain_alone_ms": {"p50": _pct(alone, 50), "p99": _pct(alone, 99),
+                           "mix_mean": round(float(np.mean(alone[:len(specs)])), 3),
+                           "big_mean": round(float(np.mean([alone[e] for e in big])), 3)},
+        "bit_identical": f"{total - mismatched}/{total}",
+        "server_stats": {k: st[k] for k in ("batches", "max_batch_seen", "max_lanes_busy",
+                                            "pool_high_water", "max_waiting")},
+        "seconds_per_rate": seconds,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +300,13 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--skip-forms", action="store_true",
+                    help="only the single-pair workload (no batched / serving measurement)")
+    ap.add_argument("--mix-pairs", type=int, default=64)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--serve-rates", default="20,50,100,200,300,400")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-lanes", type=int, default=3)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy of "
                          "known size, to calibrate rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -173,6 +400,27 @@ def main() -> int:
         cb, mism = cpu_baseline(ls, params, left, right, gpu_out, args.cpu_threads)
         line["cpu_baseline"] = cb
         line["parity_vs_oracle_mismatched_pixels"] = mism
+    if not args.skip_forms:
+        del m, d_left, d_right, d_out
+        gc.collect()
+        torch.cuda.empty_cache()
+        from lumenstereo_amd import workloads
+
+        specs = workloads.mixed_specs(args.mix_pairs, SEED + 1000)
+        imgs = [s.images() for s in specs]
+        ref = ls.StereoSGBM()
+        plain_outs = []
+        for s, (l, r) in zip(specs, imgs):
+            ref.setParams(s.params)
+            plain_outs.append(ref.compute(l, r))
+        del ref
+        gc.collect()
+        print(f"# batched: {len(specs)} pairs", file=sys.stderr, flush=True)
+        line["batched"] = bench_batched(ls, torch, specs, imgs, plain_outs, args.batch_reps)
+        print("# serving sweep", file=sys.stderr, flush=True)
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        line["serving"] = bench_serving(ls, torch, specs, imgs, plain_outs, rates,
+                                        args.serve_seconds, args.serve_lanes)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -139,7 +139,7 @@ int lumen_synth_pair(int width, int height, float dmin, float dmax, uint64_t see
 /* ------------------------------------------------------------------------------------------
  * Batched form (no reference counterpart). Many pairs of different size, disparity range, cost
  * mode and path count are processed stage by stage in persistent kernels over work lists, the
- * whole pipeline replayed as one captured hipGraph. Every pair's disparity is bit-identical to
+ * whole pipeline replayed as one hipGraph. Every pair's disparity is bit-identical to
  * lumen_sgbm_compute of the same pair with the same parameters.
  * ---------------------------------------------------------------------------------------- */
 typedef struct lumen_pair_desc {
@@ -168,6 +168,64 @@ int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs, in
 int lumen_batch_get_timings(const lumen_batch* b, float out[4]);
 
 const char* lumen_batch_last_error(const lumen_batch* b);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving form (no reference counterpart). Requests (stereo pairs of any size, range, cost mode
+ * and path count) are submitted as they arrive and return a ticket. A dispatcher thread admits
+ * them into a device pool by their working-set size (cost volume, path buffers, maps), and
+ * whenever one of the lanes (a stream with its own persistent-kernel pipeline) is free it starts
+ * every admitted request that has not started as one batch on it. So requests that arrive
+ * while others are on the GPU run alongside them. Every result is bit-identical to
+ * lumen_sgbm_compute of the same pair with the same parameters.
+ * ---------------------------------------------------------------------------------------- */
+typedef struct lumen_server_config {
+    size_t pool_bytes;   /* device pool; 0 = half of the free device memory */
+    int32_t lanes;       /* batches in flight at once, 1..4; 0 = 2 */
+    int32_t max_batch;   /* most requests one batch takes; 0 = 64 */
+    int32_t device_io;   /* 0: request images/outputs are host memory; 1: device memory */
+    int32_t max_bypass;  /* smaller requests that may pass a waiting one; 0 = 8, -1 = none */
+} lumen_server_config;
+
+typedef struct lumen_request_info {
+    int32_t status;      /* LUMEN_OK or the error code of the request */
+    int32_t lane;        /* lane it ran on */
+    int32_t batch_size;  /* requests in its batch */
+    int32_t reserved;
+    int64_t t_submit_ns; /* CLOCK_MONOTONIC times (lumen_now_ns) */
+    int64_t t_admit_ns;  /* pool range granted */
+    int64_t t_start_ns;  /* its batch enqueued */
+    int64_t t_done_ns;   /* output written */
+} lumen_request_info;
+
+typedef struct lumen_server_stats {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t max_batch_seen;
+    uint64_t max_lanes_busy;   /* most lanes that had a batch in flight at the same time */
+    uint64_t pool_bytes, pool_high_water;
+    uint64_t max_waiting;      /* most requests waiting for admission at once */
+} lumen_server_stats;
+
+typedef struct lumen_server lumen_server;
+
+/* CLOCK_MONOTONIC in ns (the clock of lumen_request_info). */
+int64_t lumen_now_ns(void);
+
+int lumen_server_create(const lumen_server_config* cfg, lumen_server** out);
+/* Finishes every submitted request, then frees the server. */
+int lumen_server_destroy(lumen_server* s);
+/* Validates the pair as lumen_sgbm_compute does and queues it; never blocks on the device.
+ * LUMEN_ENOMEM if its working set is larger than the whole pool. The images and the output must
+ * stay valid until lumen_server_wait returns for the ticket. */
+int lumen_server_submit(lumen_server* s, const lumen_pair_desc* pair, uint64_t* ticket);
+/* Waits for a ticket (timeout_us < 0: no limit) and releases it. Returns the request's status,
+ * or LUMEN_EINVAL for an unknown ticket; on timeout returns LUMEN_EDEVICE with the ticket kept.
+ * info may be NULL. */
+int lumen_server_wait(lumen_server* s, uint64_t ticket, int64_t timeout_us,
+                      lumen_request_info* info);
+/* Waits until no request is queued or running. */
+int lumen_server_drain(lumen_server* s);
+int lumen_server_get_stats(lumen_server* s, lumen_server_stats* out);
+const char* lumen_server_last_error(const lumen_server* s);
 
 #ifdef __cplusplus
 }

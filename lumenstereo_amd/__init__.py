@@ -36,6 +36,8 @@ __all__ = [
     "LIB_PATH",
     "STAGE",
     "BatchSGBM",
+    "ServeSGBM",
+    "now_ns",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "liblumen_sgbm.so"
@@ -157,6 +159,31 @@ class CPairDesc(ctypes.Structure):
                 ("left", ctypes.c_void_p), ("right", ctypes.c_void_p), ("out", ctypes.c_void_p)]
 
 
+class CServerConfig(ctypes.Structure):
+    """lumen_server_config (include/lumen_sgbm.h)."""
+
+    _fields_ = [("pool_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int32),
+                ("max_batch", ctypes.c_int32), ("device_io", ctypes.c_int32),
+                ("max_bypass", ctypes.c_int32)]
+
+
+class CRequestInfo(ctypes.Structure):
+    """lumen_request_info (include/lumen_sgbm.h)."""
+
+    _fields_ = [("status", ctypes.c_int32), ("lane", ctypes.c_int32),
+                ("batch_size", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("t_submit_ns", ctypes.c_int64), ("t_admit_ns", ctypes.c_int64),
+                ("t_start_ns", ctypes.c_int64), ("t_done_ns", ctypes.c_int64)]
+
+
+class CServerStats(ctypes.Structure):
+    """lumen_server_stats (include/lumen_sgbm.h)."""
+
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "submitted", "completed", "failed", "batches", "max_batch_seen", "max_lanes_busy",
+        "pool_bytes", "pool_high_water", "max_waiting")]
+
+
 _lib = None
 
 # Every symbol include/lumen_sgbm.h declares, with its ctypes signature.
@@ -189,6 +216,16 @@ _SIGNATURES = {
     "lumen_batch_run": (_c_int, [_c_void_p, ctypes.POINTER(CPairDesc), _c_int, _c_int]),
     "lumen_batch_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "lumen_batch_last_error": (ctypes.c_char_p, [_c_void_p]),
+    "lumen_now_ns": (ctypes.c_int64, []),
+    "lumen_server_create": (_c_int, [ctypes.POINTER(CServerConfig), ctypes.POINTER(_c_void_p)]),
+    "lumen_server_destroy": (_c_int, [_c_void_p]),
+    "lumen_server_submit": (_c_int, [_c_void_p, ctypes.POINTER(CPairDesc),
+                                     ctypes.POINTER(ctypes.c_uint64)]),
+    "lumen_server_wait": (_c_int, [_c_void_p, ctypes.c_uint64, ctypes.c_int64,
+                                   ctypes.POINTER(CRequestInfo)]),
+    "lumen_server_drain": (_c_int, [_c_void_p]),
+    "lumen_server_get_stats": (_c_int, [_c_void_p, ctypes.POINTER(CServerStats)]),
+    "lumen_server_last_error": (ctypes.c_char_p, [_c_void_p]),
 }
 
 
@@ -245,7 +282,7 @@ class StereoSGBM:
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             self._lib.lumen_sgbm_destroy(h)
-            self._h = ctypes.c_void_p()
+        self._h = None
 
     @property
     def handle(self) -> ctypes.c_void_p:
@@ -368,7 +405,7 @@ class BatchSGBM:
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             self._lib.lumen_batch_destroy(h)
-            self._h = ctypes.c_void_p()
+        self._h = None
 
     def arena_bytes(self) -> int:
         v = ctypes.c_size_t()
@@ -412,3 +449,85 @@ class BatchSGBM:
         t = np.zeros(4, np.float32)
         self._lib.lumen_batch_get_timings(self._h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
         return t
+
+
+def now_ns() -> int:
+    """CLOCK_MONOTONIC in ns, the clock of the serving form's request times."""
+    return int(library().lumen_now_ns())
+
+
+class ServeSGBM:
+    """Serving form (lumen_server_* in include/lumen_sgbm.h): requests are admitted into a
+    device pool by their working-set size and run in continuous batches over several lanes.
+    Results are bit-identical to StereoSGBM."""
+
+    def __init__(self, pool_bytes: int = 0, lanes: int = 2, max_batch: int = 64,
+                 device_io: bool = False, max_bypass: int = 8):
+        self._lib = library()
+        self._h = ctypes.c_void_p()
+        self._device_io = bool(device_io)
+        cfg = CServerConfig(int(pool_bytes), int(lanes), int(max_batch), int(device_io),
+                            int(max_bypass))
+        rc = self._lib.lumen_server_create(ctypes.byref(cfg), ctypes.byref(self._h))
+        if rc != LUMEN_OK:
+            raise GpuException(self._lib.lumen_server_last_error(None).decode(), rc)
+        self._live = {}
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._lib.lumen_server_destroy(h)
+        self._h = None
+
+    def close(self) -> None:
+        """Finishes every submitted request and frees the server."""
+        self.__del__()
+
+    def _raise(self, rc):
+        msg = self._lib.lumen_server_last_error(self._h).decode()
+        if rc == LUMEN_EINVAL:
+            raise StereoException(msg)
+        raise GpuException(msg, rc)
+
+    def submit(self, params: StereoParams, left: np.ndarray, right: np.ndarray) -> int:
+        """Queues one host-memory pair; returns its ticket. Never waits for the device."""
+        left = np.ascontiguousarray(left, np.uint8)
+        right = np.ascontiguousarray(right, np.uint8)
+        if left.shape != right.shape or left.ndim != 2:
+            raise StereoException("Left and right images must have same size")
+        out = np.empty(left.shape, np.int16)
+        d = CPairDesc(params.to_c(), left.shape[1], left.shape[0], _ptr(left), _ptr(right),
+                      _ptr(out))
+        t = ctypes.c_uint64()
+        rc = self._lib.lumen_server_submit(self._h, ctypes.byref(d), ctypes.byref(t))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        self._live[t.value] = (left, right, out)
+        return t.value
+
+    def submit_desc(self, desc: CPairDesc) -> int:
+        """Queues a pair given as a CPairDesc (device pointers when device_io is set)."""
+        t = ctypes.c_uint64()
+        rc = self._lib.lumen_server_submit(self._h, ctypes.byref(desc), ctypes.byref(t))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        return t.value
+
+    def wait(self, ticket: int, timeout_s: float | None = None):
+        """Waits for a ticket; returns (disparity or None for device requests, info dict)."""
+        info = CRequestInfo()
+        tmo = -1 if timeout_s is None else int(timeout_s * 1e6)
+        rc = self._lib.lumen_server_wait(self._h, ticket, tmo, ctypes.byref(info))
+        held = self._live.pop(ticket, None)
+        if rc != LUMEN_OK:
+            self._raise(rc)
+        d = {k: getattr(info, k) for k, _ in CRequestInfo._fields_ if k != "reserved"}
+        return (held[2] if held is not None else None), d
+
+    def drain(self) -> None:
+        self._lib.lumen_server_drain(self._h)
+
+    def stats(self) -> dict:
+        st = CServerStats()
+        self._lib.lumen_server_get_stats(self._h, ctypes.byref(st))
+        return {k: int(getattr(st, k)) for k, _ in CServerStats._fields_}

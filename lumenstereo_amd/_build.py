@@ -19,7 +19,7 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_capi.hip", "sgbm_batch.hip"]
+HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_capi.hip", "sgbm_batch.hip", "sgbm_serve.hip"]
 CXX_SOURCES = ["synth_scene.cpp"]
 
 HIP_FLAGS = [
@@ -53,7 +53,7 @@ def build_library() -> Path:
               "-o", obj])
         objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-pthread", "-o", tmp])
     os.replace(tmp, LIB)
     return LIB
 
@@ -79,10 +79,23 @@ def build_cpp_tests() -> Path:
     return CPP_TEST_BIN
 
 
+SERVE_TEST_SRC = ROOT / "tests" / "cpp" / "test_serve_core.cpp"
+SERVE_TEST_BIN = ROOT / "tests" / "cpp" / "test_serve_core"
+
+
+def build_serve_core_test() -> Path:
+    """Host-only test program of the serving loop's pool and admission logic (no HIP)."""
+    tmp = SERVE_TEST_BIN.with_suffix(".tmp")
+    _run(["g++", "-O1", "-std=c++17", "-I", CSRC, SERVE_TEST_SRC, "-o", tmp])
+    os.replace(tmp, SERVE_TEST_BIN)
+    return SERVE_TEST_BIN
+
+
 def build_all() -> None:
     build_oracle()
     build_library()
     build_cpp_tests()
+    build_serve_core_test()
 
 
 if __name__ == "__main__":

@@ -12,7 +12,7 @@
 //   median      (pair, 64x4 tile)          block-wide
 // The units are the same device bodies the per-pair kernels run (sgbm_device.h), so every
 // pair's disparity is bit-identical to the per-pair path. Grids are sized to what fits on the
-// device at once, so a captured graph does not depend on the batch it processes; the work
+// device at once, so a built graph does not depend on the batch it processes; the work
 // lists and the pair table are rewritten between replays.
 #include <hip/hip_runtime.h>
 
@@ -21,6 +21,7 @@
 #include <string>
 #include <vector>
 
+#include "batch_engine.h"
 #include "lumen_sgbm.h"
 #include "sgbm_device.h"
 #include "sgbm_host.h"
@@ -31,7 +32,7 @@ namespace {
 
 enum Counter { kCtrCost = 0, kCtrPaths, kCtrPathsBig, kCtrWta, kCtrWtaBig, kCtrSpeckle, kCtrMedian, kNumCounters };
 
-// Device-resident sizes of the current batch (so a captured graph serves any batch).
+// Device-resident sizes of the current batch (so one built graph serves any batch).
 enum HeaderField { kHdrPairs = 0, kHdrWta0, kHdrWta1, kHdrSpeckle, kHdrItems, kHdrItemsBig, kHdrFields };
 
 // Prefix tables, each with room for `cap` pairs: costStart, wta0Start, wta1Start, spkStart,
@@ -46,13 +47,6 @@ struct Tables {
     const int* wta0Pairs;
     const int* wta1Pairs;
     const int* spkPairs;
-};
-
-struct PathItem {
-    uint16_t pair;
-    uint8_t path;
-    uint8_t K;
-    int32_t s0;
 };
 
 // Prefix sums of per-pair unit counts: unit u belongs to the pair p with start[p] <= u < start[p+1].
@@ -224,77 +218,17 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const PairJob* __rest
 
 thread_local std::string g_batch_error;
 
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
 template <typename F>
 int occupancy_blocks(F kernel, size_t lds) {
     int perCU = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, 256, lds) != hipSuccess || perCU < 1)
         perCU = 1;
     int dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 1;
     return perCU * std::max(cus, 1);
 }
-
-}  // namespace
-
-struct lumen_batch {
-    std::string err;
-    bool ready = false;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
-    // Device arena for the pairs' working sets.
-    void* arena = nullptr;
-    size_t arenaBytes = 0;
-    // Device tables (grow on demand).
-    PairJob* jobs = nullptr;
-    size_t jobsCap = 0;
-    PathItem* items = nullptr;     // aggregation units of slices with K <= 12
-    size_t itemsCap = 0;
-    PathItem* itemsBig = nullptr;  // aggregation units of slices with K > 12
-    size_t itemsBigCap = 0;
-    int* tables = nullptr;         // header + prefix tables (see Tables)
-    size_t pairCap = 0;            // pairs the prefix tables have room for
-    int* ctr = nullptr;
-    uint16_t* trash = nullptr;
-    // Pinned host staging for host-memory batches.
-    uint8_t* hostIn = nullptr;
-    size_t hostInCap = 0;
-    int16_t* hostOut = nullptr;
-    size_t hostOutCap = 0;
-    // Captured pipeline, valid for the LDS sizes and table pointers it was captured with.
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    size_t capCostLds = 0, capWta0Lds = 0, capWta1Lds = 0;
-    const void* capPtrs[4] = {nullptr, nullptr, nullptr, nullptr};
-    bool capBig = false, capSpeckle = false;
-    float times[4] = {0, 0, 0, 0};  // total, upload, compute, download (ms)
-
-    int fail(int code, const std::string& m) {
-        err = m;
-        return code;
-    }
-    int hip_fail(hipError_t e, const char* what) {
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE;
-    }
-    ~lumen_batch() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (hipEvent_t e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-        void* dev[] = {arena, jobs, items, itemsBig, tables, ctr, trash};
-        for (void* p : dev)
-            if (p) (void)hipFree(p);
-        if (hostIn) (void)hipHostFree(hostIn);
-        if (hostOut) (void)hipHostFree(hostOut);
-    }
-};
-
-namespace {
 
 template <typename T>
 hipError_t grow(T*& p, size_t& cap, size_t need) {
@@ -324,16 +258,6 @@ hipError_t grow_pinned(void** p, size_t& cap, size_t need) {
     return e;
 }
 
-// Host-side plan of one sub-batch.
-struct Plan {
-    std::vector<PairJob> jobs;
-    std::vector<PathItem> items, itemsBig;
-    std::vector<int> costStart, wta0Start, wta1Start, spkStart, medStart;
-    std::vector<int> wta0Pairs, wta1Pairs, spkPairs;
-    std::vector<int> special;  // pairs whose cost needs the per-pair kernels
-    size_t costLds = 0, wta0Lds = 0, wta1Lds = 0;
-};
-
 size_t wta_lds_for(int K, int W) {
     switch (K) {
         case 4: return wta_lds_bytes<4>(W);
@@ -347,13 +271,43 @@ size_t wta_lds_for(int K, int W) {
     }
 }
 
-void build_plan(Plan& pl) {
+size_t tables_ints(size_t cap) { return kHdrFields + 5 * (cap + 1) + 3 * cap; }
+
+Tables tables_at(const int* hdr, size_t capacity) {
+    const int cap = static_cast<int>(capacity);
+    Tables t;
+    t.hdr = hdr;
+    t.costStart = t.hdr + kHdrFields;
+    t.wta0Start = t.costStart + (cap + 1);
+    t.wta1Start = t.wta0Start + (cap + 1);
+    t.spkStart = t.wta1Start + (cap + 1);
+    t.medStart = t.spkStart + (cap + 1);
+    t.wta0Pairs = t.medStart + (cap + 1);
+    t.wta1Pairs = t.wta0Pairs + cap;
+    t.spkPairs = t.wta1Pairs + cap;
+    return t;
+}
+
+int hip_code(hipError_t e) { return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE; }
+
+}  // namespace
+
+namespace lumen {
+
+void build_batch_plan(BatchPlan& pl) {
     const int n = static_cast<int>(pl.jobs.size());
+    pl.items.clear();
+    pl.itemsBig.clear();
+    pl.wta0Pairs.clear();
+    pl.wta1Pairs.clear();
+    pl.spkPairs.clear();
+    pl.special.clear();
     pl.costStart.assign(1, 0);
     pl.wta0Start.assign(1, 0);
     pl.wta1Start.assign(1, 0);
     pl.spkStart.assign(1, 0);
     pl.medStart.assign(1, 0);
+    pl.costLds = pl.wta0Lds = pl.wta1Lds = 0;
     for (int p = 0; p < n; ++p) {
         const PairJob& j = pl.jobs[p];
         int cost = 0;
@@ -406,7 +360,228 @@ void build_plan(Plan& pl) {
     std::stable_sort(pl.itemsBig.begin(), pl.itemsBig.end(), byLen);
 }
 
-}  // namespace
+BatchEngine::~BatchEngine() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (done_) (void)hipEventDestroy(done_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    void* dev[] = {jobs_, items_, itemsBig_, tables_, ctr_, trash_};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    if (hostTables_) (void)hipHostFree(hostTables_);
+}
+
+hipError_t BatchEngine::init() {
+    hipError_t e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&done_, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&ctr_), kNumCounters * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&trash_), kTrashElems * 2);
+    return e;
+}
+
+bool BatchEngine::idle() const { return !pending_ || hipEventQuery(done_) == hipSuccess; }
+
+// Builds (or reuses) the graph of the persistent pipeline. A built graph stays valid while
+// the device tables keep their addresses and the batch needs no more LDS than it was built
+// with; the batch's sizes are read from the device header at run time.
+int BatchEngine::ensure_graph(const BatchPlan& pl, std::string& err) {
+    auto fail = [&](hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return hip_code(e);
+    };
+    const bool big = !pl.itemsBig.empty() || !pl.wta1Pairs.empty();
+    const bool spk = !pl.spkPairs.empty();
+    const void* ptrs[4] = {jobs_, items_, itemsBig_, tables_};
+    if (exec_ && capCostLds_ >= pl.costLds && capWta0Lds_ >= pl.wta0Lds &&
+        capWta1Lds_ >= pl.wta1Lds && (capBig_ || !big) && (capSpeckle_ || !spk) &&
+        std::equal(ptrs, ptrs + 4, capPtrs_))
+        return LUMEN_OK;
+    if (exec_) {
+        (void)hipGraphExecDestroy(exec_);
+        exec_ = nullptr;
+    }
+    if (graph_) {
+        (void)hipGraphDestroy(graph_);
+        graph_ = nullptr;
+    }
+    Tables t = tables_at(tables_, pairCap_);
+    const size_t costLds = std::max<size_t>(std::max(pl.costLds, capCostLds_), 16);
+    const size_t wta0Lds = std::max<size_t>(std::max(pl.wta0Lds, capWta0Lds_), 16);
+    const size_t wta1Lds = std::max<size_t>(std::max(pl.wta1Lds, capWta1Lds_), 16);
+    const bool withBig = big || capBig_;
+    const bool withSpk = spk || capSpeckle_;
+    const int gCost = occupancy_blocks(batch_cost_kernel, costLds);
+    const int gPaths = occupancy_blocks(batch_paths_kernel<0>, 0);
+    const int gPathsBig = occupancy_blocks(batch_paths_kernel<1>, 0);
+    const int gWta0 = occupancy_blocks(batch_wta_kernel<0>, wta0Lds);
+    const int gWta1 = occupancy_blocks(batch_wta_kernel<1>, wta1Lds);
+    const int gSpk = occupancy_blocks(batch_speckle_kernel, 0);
+    const int gMed = occupancy_blocks(batch_median_kernel, 0);
+    // The graph is built node by node rather than by stream capture: a capture in progress
+    // makes device-wide synchronisation from any other thread of the process fail, and the
+    // serving loop builds graphs on its dispatcher thread while callers run.
+    hipError_t e = hipGraphCreate(&graph_, 0);
+    if (e != hipSuccess) return fail(e, "graph create");
+    hipGraphNode_t prev = nullptr;
+    auto add = [&](const void* fn, int grid, int block, size_t lds, void** args) {
+        if (e != hipSuccess) return;
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void*>(fn);
+        kp.gridDim = dim3(grid);
+        kp.blockDim = dim3(block);
+        kp.sharedMemBytes = static_cast<unsigned>(lds);
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        e = hipGraphAddKernelNode(&node, graph_, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+        prev = node;
+    };
+    void* aReset[] = {&ctr_};
+    void* aStage[] = {&jobs_, &t, &ctr_};
+    void* aPaths[] = {&jobs_, &items_, &t, &ctr_, &trash_};
+    void* aPathsBig[] = {&jobs_, &itemsBig_, &t, &ctr_, &trash_};
+    add(reinterpret_cast<const void*>(&reset_counters_kernel), 1, 64, 0, aReset);
+    add(reinterpret_cast<const void*>(&batch_cost_kernel), gCost, 256, costLds, aStage);
+    add(reinterpret_cast<const void*>(&batch_paths_kernel<0>), gPaths, 256, 0, aPaths);
+    if (withBig)
+        add(reinterpret_cast<const void*>(&batch_paths_kernel<1>), gPathsBig, 256, 0, aPathsBig);
+    add(reinterpret_cast<const void*>(&batch_wta_kernel<0>), gWta0, 256, wta0Lds, aStage);
+    if (withBig) add(reinterpret_cast<const void*>(&batch_wta_kernel<1>), gWta1, 256, wta1Lds, aStage);
+    if (withSpk) add(reinterpret_cast<const void*>(&batch_speckle_kernel), gSpk, 256, 0, aStage);
+    add(reinterpret_cast<const void*>(&batch_median_kernel), gMed, 256, 0, aStage);
+    if (e != hipSuccess) return fail(e, "graph kernel node");
+    e = hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail(e, "graph instantiate");
+    capCostLds_ = costLds;
+    capWta0Lds_ = wta0Lds;
+    capWta1Lds_ = wta1Lds;
+    capBig_ = withBig;
+    capSpeckle_ = withSpk;
+    std::copy(ptrs, ptrs + 4, capPtrs_);
+    return LUMEN_OK;
+}
+
+int BatchEngine::enqueue(BatchPlan& pl, std::string& err) {
+    auto fail = [&](hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return hip_code(e);
+    };
+    const int npairs = static_cast<int>(pl.jobs.size());
+    if (npairs == 0) return LUMEN_OK;
+    if (npairs > 65535) {
+        err = "at most 65535 pairs per batch";
+        return LUMEN_EINVAL;
+    }
+    // The pinned mirror and the device tables are rewritten below.
+    if (pending_) {
+        hipError_t e = hipEventSynchronize(done_);
+        if (e != hipSuccess) return fail(e, "previous batch");
+        pending_ = false;
+    }
+    build_batch_plan(pl);
+    // Tables grow (and move, which forces a new capture) only when a batch outgrows them.
+    hipError_t e = grow(jobs_, jobsCap_, npairs);
+    if (e == hipSuccess) e = grow(items_, itemsCap_, pl.items.size());
+    if (e == hipSuccess) e = grow(itemsBig_, itemsBigCap_, pl.itemsBig.size());
+    if (e == hipSuccess && (size_t)npairs > pairCap_) {
+        if (tables_) (void)hipFree(tables_);
+        tables_ = nullptr;
+        const size_t cap = std::max<size_t>(npairs, 64);
+        e = hipMalloc(reinterpret_cast<void**>(&tables_), tables_ints(cap) * sizeof(int));
+        pairCap_ = e == hipSuccess ? cap : 0;
+    }
+    if (e != hipSuccess) return fail(e, "batch tables");
+    const size_t cap = pairCap_;
+    const size_t jobBytes = npairs * sizeof(PairJob);
+    const size_t itemBytes = pl.items.size() * sizeof(PathItem);
+    const size_t bigBytes = pl.itemsBig.size() * sizeof(PathItem);
+    const size_t tabBytes = tables_ints(cap) * sizeof(int);
+    const size_t offItems = align_up(jobBytes, 256);
+    const size_t offBig = offItems + align_up(itemBytes, 256);
+    const size_t offTab = offBig + align_up(bigBytes, 256);
+    e = grow_pinned(&hostTables_, hostTablesCap_, offTab + tabBytes);
+    if (e != hipSuccess) return fail(e, "pinned tables");
+    char* h = static_cast<char*>(hostTables_);
+    std::memcpy(h, pl.jobs.data(), jobBytes);
+    if (itemBytes) std::memcpy(h + offItems, pl.items.data(), itemBytes);
+    if (bigBytes) std::memcpy(h + offBig, pl.itemsBig.data(), bigBytes);
+    int* tab = reinterpret_cast<int*>(h + offTab);
+    tab[kHdrPairs] = npairs;
+    tab[kHdrWta0] = static_cast<int>(pl.wta0Pairs.size());
+    tab[kHdrWta1] = static_cast<int>(pl.wta1Pairs.size());
+    tab[kHdrSpeckle] = static_cast<int>(pl.spkPairs.size());
+    tab[kHdrItems] = static_cast<int>(pl.items.size());
+    tab[kHdrItemsBig] = static_cast<int>(pl.itemsBig.size());
+    int* base = tab + kHdrFields;
+    auto put = [&](const std::vector<int>& v, size_t n) {
+        for (size_t i = 0; i < n; ++i) base[i] = i < v.size() ? v[i] : (v.empty() ? 0 : v.back());
+        base += n;
+    };
+    put(pl.costStart, cap + 1);
+    put(pl.wta0Start, cap + 1);
+    put(pl.wta1Start, cap + 1);
+    put(pl.spkStart, cap + 1);
+    put(pl.medStart, cap + 1);
+    put(pl.wta0Pairs, cap);
+    put(pl.wta1Pairs, cap);
+    put(pl.spkPairs, cap);
+    hipStream_t s = stream_;
+    e = hipMemcpyAsync(jobs_, h, jobBytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && itemBytes)
+        e = hipMemcpyAsync(items_, h + offItems, itemBytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && bigBytes)
+        e = hipMemcpyAsync(itemsBig_, h + offBig, bigBytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(tables_, tab, tabBytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return fail(e, "table upload");
+    // Pairs whose SAD window is outside the LDS kernel's range get per-pair cost launches.
+    for (int p : pl.special) {
+        e = launch_cost(pl.jobs[p], s);
+        if (e != hipSuccess) return fail(e, "per-pair cost launch");
+    }
+    int rc = ensure_graph(pl, err);
+    if (rc) return rc;
+    e = hipGraphLaunch(exec_, s);
+    if (e != hipSuccess) return fail(e, "graph launch");
+    e = hipEventRecord(done_, s);
+    if (e != hipSuccess) return fail(e, "event record");
+    pending_ = true;
+    return LUMEN_OK;
+}
+
+}  // namespace lumen
+
+struct lumen_batch {
+    std::string err;
+    BatchEngine engine;
+    hipEvent_t ev[4] = {};
+    // Device arena for the pairs' working sets.
+    void* arena = nullptr;
+    size_t arenaBytes = 0;
+    // Pinned host staging for host-memory batches.
+    uint8_t* hostIn = nullptr;
+    size_t hostInCap = 0;
+    int16_t* hostOut = nullptr;
+    size_t hostOutCap = 0;
+    float times[4] = {0, 0, 0, 0};  // total, upload, compute, download (ms)
+
+    int fail(int code, const std::string& m) {
+        err = m;
+        return code;
+    }
+    int hip_fail(hipError_t e, const char* what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return hip_code(e);
+    }
+    ~lumen_batch() {
+        if (engine.stream()) (void)hipStreamSynchronize(engine.stream());
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (arena) (void)hipFree(arena);
+        if (hostIn) (void)hipHostFree(hostIn);
+        if (hostOut) (void)hipHostFree(hostOut);
+    }
+};
 
 extern "C" {
 
@@ -415,7 +590,7 @@ int lumen_batch_create(size_t arena_bytes, lumen_batch** out) {
     *out = nullptr;
     lumen_batch* b = new (std::nothrow) lumen_batch();
     if (!b) return LUMEN_ENOMEM;
-    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    hipError_t e = b->engine.init();
     for (int i = 0; e == hipSuccess && i < 4; ++i) e = hipEventCreate(&b->ev[i]);
     if (e == hipSuccess && arena_bytes == 0) {
         size_t freeB = 0, totalB = 0;
@@ -424,14 +599,11 @@ int lumen_batch_create(size_t arena_bytes, lumen_batch** out) {
     }
     if (e == hipSuccess) e = hipMalloc(&b->arena, arena_bytes);
     if (e == hipSuccess) b->arenaBytes = arena_bytes;
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->ctr), kNumCounters * sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&b->trash), kTrashElems * 2);
     if (e != hipSuccess) {
         g_batch_error = std::string("lumen_batch_create: ") + hipGetErrorString(e);
         delete b;
-        return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE;
+        return hip_code(e);
     }
-    b->ready = true;
     *out = b;
     return LUMEN_OK;
 }
@@ -457,154 +629,7 @@ int lumen_batch_arena_bytes(const lumen_batch* b, size_t* out) {
     return LUMEN_OK;
 }
 
-}  // extern "C"
-
-namespace {
-
-Tables tables_of(const lumen_batch* b) {
-    const int cap = static_cast<int>(b->pairCap);
-    Tables t;
-    t.hdr = b->tables;
-    t.costStart = t.hdr + kHdrFields;
-    t.wta0Start = t.costStart + (cap + 1);
-    t.wta1Start = t.wta0Start + (cap + 1);
-    t.spkStart = t.wta1Start + (cap + 1);
-    t.medStart = t.spkStart + (cap + 1);
-    t.wta0Pairs = t.medStart + (cap + 1);
-    t.wta1Pairs = t.wta0Pairs + cap;
-    t.spkPairs = t.wta1Pairs + cap;
-    return t;
-}
-
-size_t tables_ints(size_t cap) { return kHdrFields + 5 * (cap + 1) + 3 * cap; }
-
-// Captures (or reuses) the graph of the persistent pipeline. A captured graph stays valid while
-// the device tables keep their addresses and the batch needs no more LDS than it was captured
-// with; the batch's sizes are read from the device header at run time.
-int ensure_graph(lumen_batch* b, const Plan& pl) {
-    const bool big = !pl.itemsBig.empty() || !pl.wta1Pairs.empty();
-    const bool spk = !pl.spkPairs.empty();
-    const void* ptrs[4] = {b->jobs, b->items, b->itemsBig, b->tables};
-    if (b->exec && b->capCostLds >= pl.costLds && b->capWta0Lds >= pl.wta0Lds &&
-        b->capWta1Lds >= pl.wta1Lds && (b->capBig || !big) && (b->capSpeckle || !spk) &&
-        std::equal(ptrs, ptrs + 4, b->capPtrs))
-        return LUMEN_OK;
-    if (b->exec) {
-        (void)hipGraphExecDestroy(b->exec);
-        b->exec = nullptr;
-    }
-    if (b->graph) {
-        (void)hipGraphDestroy(b->graph);
-        b->graph = nullptr;
-    }
-    const Tables t = tables_of(b);
-    const size_t costLds = std::max<size_t>(std::max(pl.costLds, b->capCostLds), 16);
-    const size_t wta0Lds = std::max<size_t>(std::max(pl.wta0Lds, b->capWta0Lds), 16);
-    const size_t wta1Lds = std::max<size_t>(std::max(pl.wta1Lds, b->capWta1Lds), 16);
-    const bool withBig = big || b->capBig;
-    const bool withSpk = spk || b->capSpeckle;
-    const int gCost = occupancy_blocks(batch_cost_kernel, costLds);
-    const int gPaths = occupancy_blocks(batch_paths_kernel<0>, 0);
-    const int gPathsBig = occupancy_blocks(batch_paths_kernel<1>, 0);
-    const int gWta0 = occupancy_blocks(batch_wta_kernel<0>, wta0Lds);
-    const int gWta1 = occupancy_blocks(batch_wta_kernel<1>, wta1Lds);
-    const int gSpk = occupancy_blocks(batch_speckle_kernel, 0);
-    const int gMed = occupancy_blocks(batch_median_kernel, 0);
-    hipStream_t s = b->stream;
-    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return b->hip_fail(e, "begin capture");
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, b->ctr);
-    hipLaunchKernelGGL(batch_cost_kernel, dim3(gCost), dim3(256), costLds, s, b->jobs, t, b->ctr);
-    hipLaunchKernelGGL(batch_paths_kernel<0>, dim3(gPaths), dim3(256), 0, s, b->jobs, b->items, t,
-                       b->ctr, b->trash);
-    if (withBig)
-        hipLaunchKernelGGL(batch_paths_kernel<1>, dim3(gPathsBig), dim3(256), 0, s, b->jobs,
-                           b->itemsBig, t, b->ctr, b->trash);
-    hipLaunchKernelGGL(batch_wta_kernel<0>, dim3(gWta0), dim3(256), wta0Lds, s, b->jobs, t, b->ctr);
-    if (withBig)
-        hipLaunchKernelGGL(batch_wta_kernel<1>, dim3(gWta1), dim3(256), wta1Lds, s, b->jobs, t, b->ctr);
-    if (withSpk)
-        hipLaunchKernelGGL(batch_speckle_kernel, dim3(gSpk), dim3(256), 0, s, b->jobs, t, b->ctr);
-    hipLaunchKernelGGL(batch_median_kernel, dim3(gMed), dim3(256), 0, s, b->jobs, t, b->ctr);
-    hipError_t le = hipGetLastError();
-    e = hipStreamEndCapture(s, &b->graph);
-    if (le != hipSuccess) return b->hip_fail(le, "captured launch");
-    if (e != hipSuccess) return b->hip_fail(e, "end capture");
-    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) return b->hip_fail(e, "graph instantiate");
-    b->capCostLds = costLds;
-    b->capWta0Lds = wta0Lds;
-    b->capWta1Lds = wta1Lds;
-    b->capBig = withBig;
-    b->capSpeckle = withSpk;
-    std::copy(ptrs, ptrs + 4, b->capPtrs);
-    return LUMEN_OK;
-}
-
-// Runs one sub-batch whose jobs already point at device images and outputs.
-int run_plan(lumen_batch* b, Plan& pl) {
-    const int npairs = static_cast<int>(pl.jobs.size());
-    build_plan(pl);
-    hipStream_t s = b->stream;
-    // Tables grow (and move, which forces a new capture) only when a batch outgrows them.
-    hipError_t e = grow(b->jobs, b->jobsCap, npairs);
-    if (e == hipSuccess) e = grow(b->items, b->itemsCap, pl.items.size());
-    if (e == hipSuccess) e = grow(b->itemsBig, b->itemsBigCap, pl.itemsBig.size());
-    if (e == hipSuccess && (size_t)npairs > b->pairCap) {
-        if (b->tables) (void)hipFree(b->tables);
-        b->tables = nullptr;
-        const size_t cap = std::max<size_t>(npairs, 64);
-        e = hipMalloc(reinterpret_cast<void**>(&b->tables), tables_ints(cap) * sizeof(int));
-        b->pairCap = e == hipSuccess ? cap : 0;
-    }
-    if (e != hipSuccess) return b->hip_fail(e, "batch tables");
-    const int cap = static_cast<int>(b->pairCap);
-    std::vector<int> tab(tables_ints(cap), 0);
-    tab[kHdrPairs] = npairs;
-    tab[kHdrWta0] = static_cast<int>(pl.wta0Pairs.size());
-    tab[kHdrWta1] = static_cast<int>(pl.wta1Pairs.size());
-    tab[kHdrSpeckle] = static_cast<int>(pl.spkPairs.size());
-    tab[kHdrItems] = static_cast<int>(pl.items.size());
-    tab[kHdrItemsBig] = static_cast<int>(pl.itemsBig.size());
-    int* base = tab.data() + kHdrFields;
-    auto put = [&](const std::vector<int>& v, int n) {
-        for (int i = 0; i < n; ++i) base[i] = i < (int)v.size() ? v[i] : (v.empty() ? 0 : v.back());
-        base += n;
-    };
-    put(pl.costStart, cap + 1);
-    put(pl.wta0Start, cap + 1);
-    put(pl.wta1Start, cap + 1);
-    put(pl.spkStart, cap + 1);
-    put(pl.medStart, cap + 1);
-    put(pl.wta0Pairs, cap);
-    put(pl.wta1Pairs, cap);
-    put(pl.spkPairs, cap);
-    e = hipMemcpyAsync(b->jobs, pl.jobs.data(), npairs * sizeof(PairJob), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && !pl.items.empty())
-        e = hipMemcpyAsync(b->items, pl.items.data(), pl.items.size() * sizeof(PathItem),
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess && !pl.itemsBig.empty())
-        e = hipMemcpyAsync(b->itemsBig, pl.itemsBig.data(), pl.itemsBig.size() * sizeof(PathItem),
-                           hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->tables, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return b->hip_fail(e, "table upload");
-    // Pairs whose SAD window is outside the LDS kernel's range get per-pair cost launches.
-    for (int p : pl.special) {
-        e = launch_cost(pl.jobs[p], s);
-        if (e != hipSuccess) return b->hip_fail(e, "per-pair cost launch");
-    }
-    int rc = ensure_graph(b, pl);
-    if (rc) return rc;
-    e = hipGraphLaunch(b->exec, s);
-    if (e != hipSuccess) return b->hip_fail(e, "graph launch");
-    return LUMEN_OK;
-}
-
-}  // namespace
-
-extern "C" int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs,
-                               int device_io) {
+int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs, int device_io) {
     if (!b) return LUMEN_EINVAL;
     b->err.clear();
     if (npairs < 0 || (npairs > 0 && !pairs)) return b->fail(LUMEN_EINVAL, "no pairs");
@@ -614,17 +639,17 @@ extern "C" int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int
         const std::string msg = validate_pair(d.params, d.width, d.height, d.left && d.right);
         if (!msg.empty()) return b->fail(LUMEN_EINVAL, "pair " + std::to_string(i) + ": " + msg);
         if (!d.out) return b->fail(LUMEN_EINVAL, "pair " + std::to_string(i) + ": output is NULL");
-        if (pair_footprint(d.params, d.width, d.height) + 3 * 256 > b->arenaBytes)
+        if (pair_region_bytes(d.params, d.width, d.height) > b->arenaBytes)
             return b->fail(LUMEN_ENOMEM, "pair " + std::to_string(i) + " does not fit the arena");
     }
-    hipStream_t s = b->stream;
+    hipStream_t s = b->engine.stream();
     hipError_t e = hipEventRecord(b->ev[0], s);
     if (e != hipSuccess) return b->hip_fail(e, "event");
     float tUp = 0, tComp = 0, tDown = 0;
     int first = 0;
     while (first < npairs) {
         // Greedy sub-batch: consecutive pairs while their working sets fit the arena.
-        Plan pl;
+        BatchPlan pl;
         size_t used = 0;
         int last = first;
         std::vector<size_t> inOff, outOff;
@@ -633,28 +658,17 @@ extern "C" int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int
         while (last < npairs) {
             const lumen_pair_desc& d = pairs[last];
             const size_t n = (size_t)d.width * d.height;
-            const size_t need = align_up(pair_footprint(d.params, d.width, d.height), 256) + 8 * 256;
+            const size_t need = pair_region_bytes(d.params, d.width, d.height);
             if (used + need > b->arenaBytes) break;
             PairJob j{};
             fill_job(j, d.params, d.width, d.height);
-            auto take = [&](size_t bytes) {
-                char* p = base + used;
-                used += align_up(bytes, 256);
-                return p;
-            };
-            uint8_t* dl = reinterpret_cast<uint8_t*>(take(n));
-            uint8_t* dr = reinterpret_cast<uint8_t*>(take(n));
-            j.left = device_io ? d.left : dl;
-            j.right = device_io ? d.right : dr;
-            j.cost = reinterpret_cast<uint8_t*>(take(j.pathStride));
-            j.L = reinterpret_cast<uint16_t*>(take(j.pathStride * 2 * j.npaths));
-            j.dispL = reinterpret_cast<int16_t*>(take(n * 2));
-            j.dispR = reinterpret_cast<int16_t*>(take(n * 2));
-            j.dispLR = reinterpret_cast<int16_t*>(take(n * 2));
-            j.wtaInt = reinterpret_cast<int16_t*>(take(n * 2));
-            j.spk = reinterpret_cast<int16_t*>(take(n * 2));
-            int16_t* dout = reinterpret_cast<int16_t*>(take(n * 2));
-            j.out = device_io ? d.out : dout;
+            layout_pair(j, base + used);
+            used += need;
+            if (device_io) {
+                j.left = d.left;
+                j.right = d.right;
+                j.out = d.out;
+            }
             pl.jobs.push_back(j);
             inOff.push_back(inBytes);
             inBytes += 2 * n;
@@ -693,7 +707,7 @@ extern "C" int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int
         }
         e = hipEventRecord(b->ev[2], s);
         if (e != hipSuccess) return b->hip_fail(e, "event");
-        int rc = run_plan(b, pl);
+        int rc = b->engine.enqueue(pl, b->err);
         if (rc) return rc;
         e = hipEventRecord(b->ev[3], s);
         if (e != hipSuccess) return b->hip_fail(e, "event");
@@ -730,3 +744,5 @@ extern "C" int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int
     b->times[3] = tDown;
     return LUMEN_OK;
 }
+
+}  // extern "C"

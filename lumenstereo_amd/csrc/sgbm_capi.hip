@@ -53,6 +53,36 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
+// Pinned host staging: a pageable strided copy goes through the runtime's own staging row by
+// row, which costs more than the whole frame's compute at 1080p.
+struct PinnedBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && ptr) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipHostFree(ptr);
+            ptr = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipHostMalloc(&ptr, bytes, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = bytes;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
 constexpr int kNumEvents = 6;  // start, uploaded, cost, paths, wta, post
 constexpr int kRing = 64;
 
@@ -77,6 +107,7 @@ struct lumen_sgbm {
     bool haveResult = false;
     DevBuf left, right, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
         sum, trash;
+    PinnedBuf stageIn, stageOut;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -111,6 +142,8 @@ struct lumen_sgbm {
         DevBuf* all[] = {&left, &right, &cost, &paths, &dispL,
                          &dispR, &dispLR, &wtaInt, &spk, &out, &sum, &trash};
         for (DevBuf* b : all) b->release();
+        stageIn.release();
+        stageOut.release();
     }
 };
 
@@ -313,33 +346,45 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
     if (left_stride == 0) left_stride = (size_t)W * ch;
     if (right_stride == 0) right_stride = (size_t)W * ch;
     if (out_stride == 0) out_stride = (size_t)W * 2;
-    std::vector<uint8_t> grayL, grayR;
+    const size_t n = (size_t)W * H;
+    hipStream_t s = h->stream;
+    // The previous call's download may still be reading the staging buffers only if it failed
+    // half-way; wait for the stream before the host writes them.
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return h->hip_fail(e, "stream");
+    e = h->stageIn.ensure(2 * n);
+    if (e != hipSuccess) return h->hip_fail(e, "pinned input staging");
+    e = h->stageOut.ensure(2 * n);
+    if (e != hipSuccess) return h->hip_fail(e, "pinned output staging");
+    uint8_t* pin = h->stageIn.as<uint8_t>();
     if (ch == 3) {
-        grayL.resize((size_t)W * H);
-        grayR.resize((size_t)W * H);
-        bgr_to_gray(left, left_stride, W, H, grayL.data());
-        bgr_to_gray(right, right_stride, W, H, grayR.data());
-        left = grayL.data();
-        right = grayR.data();
-        left_stride = right_stride = (size_t)W;
+        bgr_to_gray(left, left_stride, W, H, pin);
+        bgr_to_gray(right, right_stride, W, H, pin + n);
+    } else {
+        for (int y = 0; y < H; ++y) {
+            std::memcpy(pin + (size_t)y * W, left + (size_t)y * left_stride, W);
+            std::memcpy(pin + n + (size_t)y * W, right + (size_t)y * right_stride, W);
+        }
     }
     const int slot = begin_slot(h);
-    hipStream_t s = h->stream;
-    hipError_t e = hipEventRecord(h->ev[slot][0], s);
+    e = hipEventRecord(h->ev[slot][0], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
-    e = hipMemcpy2DAsync(h->left.ptr, W, left, left_stride, W, H, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(h->left.ptr, pin, n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return h->hip_fail(e, "upload left");
-    e = hipMemcpy2DAsync(h->right.ptr, W, right, right_stride, W, H, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(h->right.ptr, pin + n, n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return h->hip_fail(e, "upload right");
     e = hipEventRecord(h->ev[slot][1], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
     rc = run_stages(h, h->left.as<uint8_t>(), h->right.as<uint8_t>(), h->out.as<int16_t>(), slot);
     if (rc) return rc;
-    e = hipMemcpy2DAsync(out, out_stride, h->out.ptr, (size_t)W * 2, (size_t)W * 2, H,
-                         hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(h->stageOut.ptr, h->out.ptr, n * 2, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return h->hip_fail(e, "download");
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return h->hip_fail(e, "compute");
+    const int16_t* pout = h->stageOut.as<int16_t>();
+    for (int y = 0; y < H; ++y)
+        std::memcpy(reinterpret_cast<uint8_t*>(out) + (size_t)y * out_stride, pout + (size_t)y * W,
+                    (size_t)W * 2);
     h->haveResult = true;
     return LUMEN_OK;
 }

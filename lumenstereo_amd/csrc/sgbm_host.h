@@ -51,6 +51,41 @@ inline size_t pair_footprint(const lumen_stereo_params& p, int W, int H) {
     return vol + vol * 2 * num_paths(p) + n * 2 + n * 2 * 6;
 }
 
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Bytes of one pair's working set laid out as layout_pair() does: the pieces of
+// pair_footprint(), each rounded up to 256 bytes.
+inline size_t pair_region_bytes(const lumen_stereo_params& p, int W, int H) {
+    const size_t n = (size_t)W * H;
+    const size_t vol = n * padded_range(p.maxDisparity - p.minDisparity);
+    return 2 * align_up(n, 256) + align_up(vol, 256) + align_up(vol * 2 * num_paths(p), 256) +
+           6 * align_up(n * 2, 256);
+}
+
+// Points a filled PairJob's buffers into a region of pair_region_bytes() at `base` (256-byte
+// aligned): left, right, cost, paths, dispL, dispR, dispLR, wtaInt, spk, then the pair's own
+// output map, whose address is returned (the job's `out` is set to it too).
+inline int16_t* layout_pair(PairJob& j, char* base) {
+    const size_t n = (size_t)j.W * j.H;
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        char* q = base + used;
+        used += align_up(bytes, 256);
+        return q;
+    };
+    j.left = reinterpret_cast<uint8_t*>(take(n));
+    j.right = reinterpret_cast<uint8_t*>(take(n));
+    j.cost = reinterpret_cast<uint8_t*>(take(j.pathStride));
+    j.L = reinterpret_cast<uint16_t*>(take(j.pathStride * 2 * j.npaths));
+    j.dispL = reinterpret_cast<int16_t*>(take(n * 2));
+    j.dispR = reinterpret_cast<int16_t*>(take(n * 2));
+    j.dispLR = reinterpret_cast<int16_t*>(take(n * 2));
+    j.wtaInt = reinterpret_cast<int16_t*>(take(n * 2));
+    j.spk = reinterpret_cast<int16_t*>(take(n * 2));
+    j.out = reinterpret_cast<int16_t*>(take(n * 2));
+    return j.out;
+}
+
 // Validation of src/sgbm_gpu.cu:148-166 plus the limits of this implementation; returns ""
 // when the pair is accepted, otherwise the message the reference (or this build) gives.
 inline std::string validate_pair(const lumen_stereo_params& p, int W, int H, bool haveImages) {

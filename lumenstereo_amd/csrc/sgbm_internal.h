@@ -48,7 +48,7 @@ hipError_t launch_wta(const PairJob& j, hipStream_t s);
 hipError_t launch_post(const PairJob& j, hipStream_t s);
 hipError_t launch_path_sum(const uint16_t* L, size_t pathStride, int npaths, uint16_t* S,
                            size_t n, hipStream_t s);
-// Fills a PairJob's scalar fields from the parameters (pointers are left untouched).
+// Number of scanlines of SGM path `path` (0..7) over a W x H image.
 int scanline_count(int path, int W, int H);
 
 }  // namespace lumen

@@ -1,0 +1,457 @@
+// sgbm_serve.hip - serving form: requests admitted into a device pool by footprint and run in
+// continuous batches over several lanes.
+//
+// One dispatcher thread owns the device side. Each pass it
+//   1. admits waiting requests whose working set fits the pool (serve_core.h Admission),
+//   2. retires lanes whose batch has finished (outputs copied out, pool ranges released),
+//   3. starts every admitted request that has not started, as one batch, on each idle lane.
+// A lane is a stream with its own BatchEngine (persistent per-stage kernels in one hipGraph), so
+// batches on different lanes overlap on the device, and requests that arrive while a batch runs
+// start on the next free lane instead of waiting for it. Each pair's result is the batched
+// pipeline's, which is bit-identical to the per-pair path.
+#include <hip/hip_runtime.h>
+#include <time.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch_engine.h"
+#include "lumen_sgbm.h"
+#include "serve_core.h"
+#include "sgbm_host.h"
+
+using namespace lumen;
+
+namespace {
+
+thread_local std::string g_serve_error;
+
+int64_t monotonic_ns() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return static_cast<int64_t>(ts.tv_sec) * 1000000000LL + ts.tv_nsec;
+}
+
+int hip_code(hipError_t e) { return e == hipErrorOutOfMemory ? LUMEN_ENOMEM : LUMEN_EDEVICE; }
+
+struct HipBacking : PoolBacking {
+    void* acquire(size_t bytes) override {
+        void* p = nullptr;
+        return hipMalloc(&p, bytes) == hipSuccess ? p : nullptr;
+    }
+    void release(void* p) override { (void)hipFree(p); }
+};
+
+struct Request {
+    uint64_t id = 0;
+    lumen_pair_desc d{};
+    size_t bytes = 0;
+    char* region = nullptr;
+    int16_t* devOut = nullptr;  // the region's output map (host-memory requests)
+    size_t inOff = 0, outOff = 0;
+    int status = LUMEN_OK;
+    std::string err;
+    lumen_request_info info{};
+    bool done = false;
+};
+
+struct Lane {
+    int index = 0;
+    BatchEngine engine;
+    hipEvent_t fin = nullptr;
+    bool busy = false;
+    std::vector<Request*> reqs;
+    std::string err;
+    void* pinIn = nullptr;
+    size_t pinInCap = 0;
+    void* pinOut = nullptr;
+    size_t pinOutCap = 0;
+    ~Lane() {
+        if (engine.stream()) (void)hipStreamSynchronize(engine.stream());
+        if (fin) (void)hipEventDestroy(fin);
+        if (pinIn) (void)hipHostFree(pinIn);
+        if (pinOut) (void)hipHostFree(pinOut);
+    }
+};
+
+hipError_t grow_pinned(void*& p, size_t& cap, size_t need) {
+    if (need <= cap && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t n = need * 5 / 4 + 4096;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e == hipSuccess) cap = n;
+    return e;
+}
+
+}  // namespace
+
+struct lumen_server {
+    lumen_server_config cfg{};
+    int device = 0;
+    HipBacking backing;
+    DevicePool pool;
+    Admission admission;
+    std::vector<std::unique_ptr<Lane>> lanes;
+    std::mutex m;
+    std::condition_variable wake;    // dispatcher: new work or shutdown
+    std::condition_variable doneCv;  // waiters: a request finished
+    std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::deque<Request*> ready;      // admitted, not started, in admission order
+    uint64_t nextId = 1;
+    bool stop = false;
+    int busyLanes = 0;
+    lumen_server_stats stats{};
+    std::thread thread;
+
+    bool idle_locked() const { return admission.empty() && ready.empty() && busyLanes == 0; }
+};
+
+namespace {
+
+// Enqueues one batch on a lane: uploads (host requests), the pipeline, downloads, the finish
+// event. Runs without the server lock; the lane and its requests belong to the dispatcher.
+int start_lane(lumen_server* S, Lane& L) {
+    const bool hostIo = S->cfg.device_io == 0;
+    hipStream_t s = L.engine.stream();
+    BatchPlan pl;
+    pl.jobs.reserve(L.reqs.size());
+    size_t inBytes = 0, outBytes = 0;
+    const int64_t t = monotonic_ns();
+    for (Request* r : L.reqs) {
+        PairJob j{};
+        fill_job(j, r->d.params, r->d.width, r->d.height);
+        r->devOut = layout_pair(j, r->region);
+        const size_t n = (size_t)j.W * j.H;
+        if (hostIo) {
+            // left and right are adjacent in the region (each rounded up to 256 bytes), so one
+            // copy uploads both from a staging block with the same gap.
+            r->inOff = inBytes;
+            inBytes += 2 * align_up(n, 256);
+            r->outOff = outBytes;
+            outBytes += align_up(n * 2, 256);
+        } else {
+            j.left = r->d.left;
+            j.right = r->d.right;
+            j.out = r->d.out;
+        }
+        pl.jobs.push_back(j);
+        r->info.t_start_ns = t;
+        r->info.lane = L.index;
+        r->info.batch_size = static_cast<int32_t>(L.reqs.size());
+    }
+    auto fail = [&](hipError_t e, const char* what) {
+        L.err = std::string(what) + ": " + hipGetErrorString(e);
+        return hip_code(e);
+    };
+    if (hostIo) {
+        hipError_t e = grow_pinned(L.pinIn, L.pinInCap, inBytes);
+        if (e == hipSuccess) e = grow_pinned(L.pinOut, L.pinOutCap, outBytes);
+        if (e != hipSuccess) return fail(e, "pinned staging");
+        uint8_t* pin = static_cast<uint8_t*>(L.pinIn);
+        for (Request* r : L.reqs) {
+            const size_t n = (size_t)r->d.width * r->d.height;
+            std::memcpy(pin + r->inOff, r->d.left, n);
+            std::memcpy(pin + r->inOff + align_up(n, 256), r->d.right, n);
+            e = hipMemcpyAsync(r->region, pin + r->inOff, align_up(n, 256) + n,
+                               hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return fail(e, "upload");
+        }
+    }
+    int rc = L.engine.enqueue(pl, L.err);
+    if (rc) return rc;
+    if (hostIo) {
+        uint8_t* pout = static_cast<uint8_t*>(L.pinOut);
+        for (Request* r : L.reqs) {
+            const size_t n = (size_t)r->d.width * r->d.height;
+            hipError_t e = hipMemcpyAsync(pout + r->outOff, r->devOut, n * 2, hipMemcpyDeviceToHost, s);
+            if (e != hipSuccess) return fail(e, "download");
+        }
+    }
+    hipError_t e = hipEventRecord(L.fin, s);
+    if (e != hipSuccess) return fail(e, "event record");
+    return LUMEN_OK;
+}
+
+// Copies a finished lane's outputs to the callers (host requests). No server lock needed.
+void finish_lane(lumen_server* S, Lane& L, hipError_t status) {
+    if (status != hipSuccess) {
+        for (Request* r : L.reqs) {
+            r->status = hip_code(status);
+            r->err = std::string("batch: ") + hipGetErrorString(status);
+        }
+        return;
+    }
+    if (S->cfg.device_io == 0) {
+        const uint8_t* pout = static_cast<const uint8_t*>(L.pinOut);
+        for (Request* r : L.reqs)
+            std::memcpy(r->d.out, pout + r->outOff, (size_t)r->d.width * r->d.height * 2);
+    }
+}
+
+void retire_locked(lumen_server* S, Lane& L) {
+    const int64_t t = monotonic_ns();
+    for (Request* r : L.reqs) {
+        S->pool.release(r->region);
+        r->region = nullptr;
+        r->info.status = r->status;
+        r->info.t_done_ns = t;
+        r->done = true;
+        if (r->status == LUMEN_OK)
+            ++S->stats.completed;
+        else
+            ++S->stats.failed;
+    }
+    L.reqs.clear();
+    L.busy = false;
+    --S->busyLanes;
+}
+
+void dispatcher(lumen_server* S) {
+    (void)hipSetDevice(S->device);
+    const size_t maxBatch = static_cast<size_t>(S->cfg.max_batch);
+    std::unique_lock<std::mutex> lk(S->m);
+    for (;;) {
+        bool progress = false;
+        for (const Admission::Admitted& a : S->admission.admit()) {
+            Request* r = S->reqs[a.id].get();
+            r->region = static_cast<char*>(a.region);
+            r->info.t_admit_ns = monotonic_ns();
+            S->ready.push_back(r);
+            progress = true;
+        }
+        S->stats.pool_high_water = S->pool.high_water();
+        for (auto& L : S->lanes) {
+            if (!L->busy) continue;
+            const hipError_t q = hipEventQuery(L->fin);
+            if (q == hipErrorNotReady) continue;
+            lk.unlock();
+            finish_lane(S, *L, q);
+            lk.lock();
+            retire_locked(S, *L);
+            S->doneCv.notify_all();
+            progress = true;
+        }
+        for (auto& L : S->lanes) {
+            if (L->busy || S->ready.empty()) continue;
+            while (!S->ready.empty() && L->reqs.size() < maxBatch) {
+                L->reqs.push_back(S->ready.front());
+                S->ready.pop_front();
+            }
+            L->busy = true;
+            ++S->busyLanes;
+            ++S->stats.batches;
+            S->stats.max_batch_seen = std::max<uint64_t>(S->stats.max_batch_seen, L->reqs.size());
+            S->stats.max_lanes_busy = std::max<uint64_t>(S->stats.max_lanes_busy, S->busyLanes);
+            lk.unlock();
+            const int rc = start_lane(S, *L);
+            if (rc) {
+                // Nothing of a failed start is left in flight once the lane's stream drains.
+                (void)hipStreamSynchronize(L->engine.stream());
+                for (Request* r : L->reqs) {
+                    r->status = rc;
+                    r->err = L->err;
+                }
+            }
+            lk.lock();
+            if (rc) {
+                retire_locked(S, *L);
+                S->doneCv.notify_all();
+            }
+            progress = true;
+        }
+        if (S->stop && S->idle_locked()) break;
+        if (!progress) {
+            // Poll the lanes often while batches run; otherwise sleep until a submit.
+            if (S->busyLanes > 0)
+                S->wake.wait_for(lk, std::chrono::microseconds(20));
+            else
+                S->wake.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t lumen_now_ns(void) { return monotonic_ns(); }
+
+int lumen_server_create(const lumen_server_config* cfg, lumen_server** out) {
+    if (!out) {
+        g_serve_error = "out is NULL";
+        return LUMEN_EINVAL;
+    }
+    *out = nullptr;
+    lumen_server_config c{};
+    if (cfg) c = *cfg;
+    if (c.lanes == 0) c.lanes = 2;
+    if (c.max_batch == 0) c.max_batch = 64;
+    if (c.max_bypass == 0) c.max_bypass = 8;
+    if (c.lanes < 1 || c.lanes > 4 || c.max_batch < 1 || c.max_batch > 65535) {
+        g_serve_error = "lanes must be 1..4 and max_batch 1..65535";
+        return LUMEN_EINVAL;
+    }
+    std::unique_ptr<lumen_server> S(new (std::nothrow) lumen_server());
+    if (!S) {
+        g_serve_error = "out of host memory";
+        return LUMEN_ENOMEM;
+    }
+    hipError_t e = hipGetDevice(&S->device);
+    if (e == hipSuccess && c.pool_bytes == 0) {
+        size_t freeB = 0, totalB = 0;
+        e = hipMemGetInfo(&freeB, &totalB);
+        c.pool_bytes = freeB / 2;
+    }
+    if (e != hipSuccess) {
+        g_serve_error = std::string("lumen_server_create: ") + hipGetErrorString(e);
+        return hip_code(e);
+    }
+    S->cfg = c;
+    if (!S->pool.open(&S->backing, c.pool_bytes)) {
+        g_serve_error = "lumen_server_create: cannot allocate a device pool of " +
+                        std::to_string(c.pool_bytes) + " bytes";
+        return LUMEN_ENOMEM;
+    }
+    S->admission = Admission(&S->pool, c.max_bypass < 0 ? 0 : c.max_bypass);
+    S->stats.pool_bytes = S->pool.capacity();
+    for (int i = 0; i < c.lanes; ++i) {
+        std::unique_ptr<Lane> L(new Lane());
+        L->index = i;
+        e = L->engine.init();
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&L->fin, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            g_serve_error = std::string("lumen_server_create: ") + hipGetErrorString(e);
+            return hip_code(e);
+        }
+        S->lanes.push_back(std::move(L));
+    }
+    lumen_server* raw = S.get();
+    try {
+        S->thread = std::thread(dispatcher, raw);
+    } catch (...) {
+        g_serve_error = "lumen_server_create: cannot start the dispatcher thread";
+        return LUMEN_ENOMEM;
+    }
+    *out = S.release();
+    return LUMEN_OK;
+}
+
+int lumen_server_destroy(lumen_server* s) {
+    if (!s) return LUMEN_OK;
+    {
+        std::lock_guard<std::mutex> g(s->m);
+        s->stop = true;
+    }
+    s->wake.notify_all();
+    if (s->thread.joinable()) s->thread.join();
+    delete s;
+    return LUMEN_OK;
+}
+
+int lumen_server_submit(lumen_server* s, const lumen_pair_desc* pair, uint64_t* ticket) {
+    if (!s || !pair || !ticket) {
+        g_serve_error = "NULL argument";
+        return LUMEN_EINVAL;
+    }
+    const lumen_pair_desc& d = *pair;
+    const std::string msg = validate_pair(d.params, d.width, d.height, d.left && d.right);
+    if (!msg.empty()) {
+        g_serve_error = msg;
+        return LUMEN_EINVAL;
+    }
+    if (!d.out) {
+        g_serve_error = "Output buffer is NULL";
+        return LUMEN_EINVAL;
+    }
+    const size_t bytes = pair_region_bytes(d.params, d.width, d.height);
+    if (!s->pool.fits_ever(bytes)) {
+        g_serve_error = "request needs " + std::to_string(bytes) + " bytes, the pool has " +
+                        std::to_string(s->pool.capacity());
+        return LUMEN_ENOMEM;
+    }
+    std::unique_ptr<Request> r(new (std::nothrow) Request());
+    if (!r) {
+        g_serve_error = "out of host memory";
+        return LUMEN_ENOMEM;
+    }
+    r->d = d;
+    r->bytes = bytes;
+    r->info.t_submit_ns = monotonic_ns();
+    r->info.lane = -1;
+    {
+        std::lock_guard<std::mutex> g(s->m);
+        if (s->stop) {
+            g_serve_error = "server is shutting down";
+            return LUMEN_EINVAL;
+        }
+        const uint64_t id = s->nextId++;
+        r->id = id;
+        *ticket = id;
+        s->admission.push(id, bytes);
+        s->reqs[id] = std::move(r);
+        ++s->stats.submitted;
+        s->stats.max_waiting = std::max<uint64_t>(s->stats.max_waiting, s->admission.waiting());
+    }
+    s->wake.notify_one();
+    return LUMEN_OK;
+}
+
+int lumen_server_wait(lumen_server* s, uint64_t ticket, int64_t timeout_us,
+                      lumen_request_info* info) {
+    if (!s) {
+        g_serve_error = "NULL server";
+        return LUMEN_EINVAL;
+    }
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) {
+        g_serve_error = "unknown ticket";
+        return LUMEN_EINVAL;
+    }
+    Request* r = it->second.get();
+    auto isDone = [&] { return r->done; };
+    if (timeout_us < 0) {
+        s->doneCv.wait(lk, isDone);
+    } else if (!s->doneCv.wait_for(lk, std::chrono::microseconds(timeout_us), isDone)) {
+        g_serve_error = "timed out";
+        return LUMEN_EDEVICE;
+    }
+    if (info) *info = r->info;
+    const int status = r->status;
+    if (status != LUMEN_OK) g_serve_error = r->err;
+    s->reqs.erase(ticket);
+    return status;
+}
+
+int lumen_server_drain(lumen_server* s) {
+    if (!s) return LUMEN_EINVAL;
+    std::unique_lock<std::mutex> lk(s->m);
+    s->doneCv.wait(lk, [&] { return s->idle_locked(); });
+    return LUMEN_OK;
+}
+
+int lumen_server_get_stats(lumen_server* s, lumen_server_stats* out) {
+    if (!s || !out) return LUMEN_EINVAL;
+    std::lock_guard<std::mutex> g(s->m);
+    *out = s->stats;
+    out->pool_high_water = s->pool.high_water();
+    return LUMEN_OK;
+}
+
+const char* lumen_server_last_error(const lumen_server* s) {
+    (void)s;
+    return g_serve_error.c_str();
+}
+
+}  // extern "C"

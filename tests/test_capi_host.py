@@ -19,7 +19,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "lumen_sgbm.h"
 
 def declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(lumen_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(lumen_\w+)\s*\(", text, re.M)))
 
 
 def test_header_symbols_exported():
