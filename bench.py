@@ -5,7 +5,9 @@ Workload: one 1920x1080 uint8 stereo pair, D=128 (minDisparity 0), Census 5x5 co
 paths, P1=200, P2=800 with adaptive P2, WTA + 1/16-px parabola, LR check (1 px), 3x3 median,
 uniqueness 0, speckle off. Synthetic planes-plus-occluders scene (disparities in [4,124]),
 generated deterministically on the host and uploaded once: the timed region starts with the
-inputs resident in HBM. A step is one full compute of the pair.
+inputs resident in HBM. A step is one full compute of the pair. (--workload picks one of two
+other single-pair configurations for profiling: 640x480 D=64 SAD 4-path, 2880x1988 D=290
+Census 4-path.)
 
 value = MDE/s = W*H*D * steps / timed seconds / 1e6 (whole job; one GPU).
 roofline = the dominant kernel (all-paths SGM aggregation): algorithmic bytes per launch =
@@ -42,22 +44,31 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 METRIC = "MDE/s + frames/s at 1080p D=128 Census; batched pairs/s and p50/p99 vs plain"
 
-W, H, MIN_D, MAX_D = 1920, 1080, 0, 128
 SEED = 20240611
 
+# Single-pair workloads: (width, height, D, census, paths, scene disparity range, low-texture
+# fraction). The default is the metric's configuration; the others are for profiling.
+WORKLOADS = {
+    "1080p-d128-census-8": (1920, 1080, 128, True, 8, (4, 124), 0.0),
+    "480p-d64-sad-4": (640, 480, 64, False, 4, (4, 60), 0.0),
+    "2880x1988-d290-census-4": (2880, 1988, 290, True, 4, (10, 280), 0.2),
+}
+DEFAULT_WORKLOAD = "1080p-d128-census-8"
 
-def workload_params(ls):
-    return ls.StereoParams(minDisparity=MIN_D, maxDisparity=MAX_D, blockSize=5, P1=200, P2=800,
+
+def workload_params(ls, D, census, paths):
+    return ls.StereoParams(minDisparity=0, maxDisparity=D, blockSize=5, P1=200, P2=800,
                            preFilterCap=31, uniquenessRatio=0, disp12MaxDiff=1,
-                           speckleWindowSize=0, numDirections=8,
-                           matchingCostMode=ls.MatchingCostMode.Census)
+                           speckleWindowSize=0, numDirections=paths,
+                           matchingCostMode=ls.MatchingCostMode.Census if census
+                           else ls.MatchingCostMode.SAD)
 
 
-def load_traffic():
-    """HBM bytes per aggregation launch measured by rocprofv3 --pmc for this workload
+def load_traffic(workload):
+    """HBM bytes per aggregation launch measured by rocprofv3 --pmc for the default workload
     (profiles/sgm_paths_1080p_d128_census8_pmc.json), or None."""
     f = ROOT / "profiles" / "sgm_paths_1080p_d128_census8_pmc.json"
-    if not f.exists():
+    if workload != DEFAULT_WORKLOAD or not f.exists():
         return None
     try:
         return json.loads(f.read_text()).get("hbm_bytes_per_launch")
@@ -65,21 +76,22 @@ def load_traffic():
         return None
 
 
-def cpu_baseline(ls, params, left, right, gpu_out, threads):
+def cpu_baseline(ls, params, left, right, gpu_out, threads, desc):
     import oracle
 
     t0 = time.perf_counter()
     ref = oracle.sgbm_mt(params, left, right, threads)
     dt = time.perf_counter() - t0
-    de = W * H * (MAX_D - MIN_D)
+    H, W = left.shape
+    de = W * H * (params.maxDisparity - params.minDisparity)
     mismatches = int(np.count_nonzero(ref != gpu_out))
     return {
         "value": round(de / dt / 1e6, 3),
         "unit": "MDE/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one full {W}x{H} D={MAX_D} Census 8-path frame, oracle algorithm with the "
-                  f"paths on {threads} threads ({dt:.2f} s); OpenCV StereoSGBM unavailable",
+        "sample": f"one full {desc} frame, oracle algorithm with the paths on {threads} threads "
+                  f"({dt:.2f} s); OpenCV StereoSGBM unavailable",
         "seconds": round(dt, 3),
     }, mismatches
 
@@ -227,8 +239,8 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
     m = ls.StereoSGBM()
     big = []
     for k, (census, paths) in enumerate([(True, 8), (False, 4), (True, 4), (False, 8)]):
-        W, H, D = workloads.BIG
-        spec = workloads.PairSpec(W, H, workloads.pair_params(D, census, paths), SEED + 5000 + k)
+        bw, bh, bd = workloads.BIG
+        spec = workloads.PairSpec(bw, bh, workloads.pair_params(bd, census, paths), SEED + 5000 + k)
         l, r = spec.images()
         big.append(len(entries))
         entries.append((spec.params, l, r))
@@ -300,8 +312,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--skip-forms", action="store_true",
                     help="only the single-pair workload (no batched / serving measurement)")
+    ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--mix-pairs", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--serve-rates", default="20,50,100,200,300,400")
@@ -320,8 +334,9 @@ def main() -> int:
 
     import lumenstereo_amd as ls
 
-    params = workload_params(ls)
-    left, right = ls.synth_pair(W, H, 4, 124, SEED)
+    W, H, D, census, paths, (dmin, dmax), lowtex = WORKLOADS[args.workload]
+    params = workload_params(ls, D, census, paths)
+    left, right = ls.synth_pair(W, H, dmin, dmax, SEED, lowtex)
     dev = torch.device("cuda:0")
     d_left = torch.from_numpy(left).to(dev)
     d_right = torch.from_numpy(right).to(dev)
@@ -339,7 +354,6 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     stage = m.timings(ncalls=min(args.steps, 64))  # total, cost, agg, post, wta, upload
 
-    D = MAX_D - MIN_D
     de = W * H * D
     ms_per_step = elapsed * 1e3 / args.steps
     value = de * args.steps / elapsed / 1e6
@@ -372,10 +386,11 @@ def main() -> int:
         "dtype": "uint16",
         "data": "synthetic",
         "config": {
-            "workload": f"plain single pair {W}x{H} uint8, D={D}, Census 5x5, {npaths} paths, "
+            "workload": f"plain single pair {W}x{H} uint8, D={D}, "
+                        f"{'Census 5x5' if census else 'Sobel-x + SAD 5x5'}, {npaths} paths, "
                         "P1=200 P2=800 adaptive, WTA+subpixel, LR check, 3x3 median",
-            "width": W, "height": H, "disparities": D, "cost": "census", "paths": npaths,
-            "seed": SEED,
+            "width": W, "height": H, "disparities": D, "cost": "census" if census else "sad",
+            "paths": npaths, "seed": SEED,
         },
         "frames_per_s": round(1e3 / ms_per_step, 2),
         "stage_ms": {"cost": round(float(stage[1]), 4), "aggregation": round(agg_ms, 4),
@@ -388,7 +403,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": load_traffic(),
+            "traffic": load_traffic(args.workload),
             "algorithmic_bytes_per_launch": agg_bytes,
         },
         "frame_roofline_frac": round(frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -397,7 +412,8 @@ def main() -> int:
                                         "GPU; cost mode and path count not stated"},
     }
     if not args.no_cpu_baseline:
-        cb, mism = cpu_baseline(ls, params, left, right, gpu_out, args.cpu_threads)
+        desc = f"{W}x{H} D={D} {'Census' if census else 'SAD'} {npaths}-path"
+        cb, mism = cpu_baseline(ls, params, left, right, gpu_out, args.cpu_threads, desc)
         line["cpu_baseline"] = cb
         line["parity_vs_oracle_mismatched_pixels"] = mism
     if not args.skip_forms:
@@ -417,10 +433,11 @@ def main() -> int:
         gc.collect()
         print(f"# batched: {len(specs)} pairs", file=sys.stderr, flush=True)
         line["batched"] = bench_batched(ls, torch, specs, imgs, plain_outs, args.batch_reps)
-        print("# serving sweep", file=sys.stderr, flush=True)
-        rates = [float(x) for x in args.serve_rates.split(",") if x]
-        line["serving"] = bench_serving(ls, torch, specs, imgs, plain_outs, rates,
-                                        args.serve_seconds, args.serve_lanes)
+        if not args.skip_serving:
+            print("# serving sweep", file=sys.stderr, flush=True)
+            rates = [float(x) for x in args.serve_rates.split(",") if x]
+            line["serving"] = bench_serving(ls, torch, specs, imgs, plain_outs, rates,
+                                            args.serve_seconds, args.serve_lanes)
     print(json.dumps(line), flush=True)
     return 0
 
