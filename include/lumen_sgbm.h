@@ -122,6 +122,33 @@ int lumen_sgbm_dump_stage(lumen_sgbm* h, int stage, int path, void* host, size_t
  * src/sgbm_gpu.cu:359). */
 int lumen_sgbm_num_paths(const lumen_sgbm* h);
 
+/* Schedule of the per-pair path (no reference counterpart): how the work of a pair is grouped
+ * and ordered on the device, never what is computed; every valid schedule gives bit-identical
+ * results. By default each pair runs the schedule the tuning table names for its width, padded
+ * disparity range (D rounded up to 64), cost mode and path count (see lumen_tuning_load). */
+typedef struct lumen_schedule {
+    int32_t agg_prefetch;  /* aggregation cost prefetch depth: 4, 8 or 16 (16 only where a
+                              block's stores fit: padded range <= 192 or 256, 384) */
+    int32_t agg_priority;  /* aggregation wave priority: 0 none, 1 by path kind, 2 by length */
+    int32_t census_tile;   /* census cost segment width in pixels: 128, 256 or 512 */
+    int32_t reserved;
+} lumen_schedule;
+
+/* Overrides the tuned schedule for this handle (NULL restores it). Checked at compute time:
+ * a schedule the pair's range does not support fails with LUMEN_EINVAL. */
+int lumen_sgbm_set_schedule(lumen_sgbm* h, const lumen_schedule* s);
+
+/* The schedule a width x height pair would run with on this handle. */
+int lumen_sgbm_get_schedule(const lumen_sgbm* h, int width, int height, lumen_schedule* out);
+
+/* Replaces the process-wide tuning table with the file at `path` (NULL or "" = tuning.tsv next
+ * to the library, which is also read on first use). On error the table is unchanged and
+ * lumen_last_error() says why. entries_out (may be NULL) receives the number of entries. */
+int lumen_tuning_load(const char* path, int* entries_out);
+
+/* Empties the tuning table: every pair runs the default schedule. */
+int lumen_tuning_clear(void);
+
 /* Device bytes the handle needs for a width x height pair with its current parameters:
  * the padded cost volume, the per-path uint16 buffers and the image-sized buffers. */
 int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int height,

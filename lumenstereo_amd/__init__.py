@@ -38,6 +38,8 @@ __all__ = [
     "BatchSGBM",
     "ServeSGBM",
     "now_ns",
+    "tuning_load",
+    "tuning_clear",
 ]
 
 LIB_PATH = Path(__file__).resolve().parent / "liblumen_sgbm.so"
@@ -159,6 +161,13 @@ class CPairDesc(ctypes.Structure):
                 ("left", ctypes.c_void_p), ("right", ctypes.c_void_p), ("out", ctypes.c_void_p)]
 
 
+class CSchedule(ctypes.Structure):
+    """lumen_schedule (include/lumen_sgbm.h)."""
+
+    _fields_ = [("agg_prefetch", ctypes.c_int32), ("agg_priority", ctypes.c_int32),
+                ("census_tile", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class CServerConfig(ctypes.Structure):
     """lumen_server_config (include/lumen_sgbm.h)."""
 
@@ -217,6 +226,10 @@ _SIGNATURES = {
     "lumen_batch_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "lumen_batch_last_error": (ctypes.c_char_p, [_c_void_p]),
     "lumen_now_ns": (ctypes.c_int64, []),
+    "lumen_sgbm_set_schedule": (_c_int, [_c_void_p, ctypes.POINTER(CSchedule)]),
+    "lumen_sgbm_get_schedule": (_c_int, [_c_void_p, _c_int, _c_int, ctypes.POINTER(CSchedule)]),
+    "lumen_tuning_load": (_c_int, [ctypes.c_char_p, ctypes.POINTER(_c_int)]),
+    "lumen_tuning_clear": (_c_int, []),
     "lumen_server_create": (_c_int, [ctypes.POINTER(CServerConfig), ctypes.POINTER(_c_void_p)]),
     "lumen_server_destroy": (_c_int, [_c_void_p]),
     "lumen_server_submit": (_c_int, [_c_void_p, ctypes.POINTER(CPairDesc),
@@ -376,6 +389,27 @@ class StereoSGBM:
     def numPaths(self) -> int:
         return self._lib.lumen_sgbm_num_paths(self._h)
 
+    def setSchedule(self, schedule: dict | None) -> None:
+        """Overrides the tuned schedule: keys agg_prefetch, agg_priority, census_tile (None
+        restores the tuned one). Results never depend on it."""
+        if schedule is None:
+            rc = self._lib.lumen_sgbm_set_schedule(self._h, None)
+        else:
+            c = CSchedule(int(schedule["agg_prefetch"]), int(schedule["agg_priority"]),
+                          int(schedule["census_tile"]), 0)
+            rc = self._lib.lumen_sgbm_set_schedule(self._h, ctypes.byref(c))
+        if rc != LUMEN_OK:
+            self._raise(rc)
+
+    def getSchedule(self, width: int, height: int) -> dict:
+        """The schedule a width x height pair would run with."""
+        c = CSchedule()
+        rc = self._lib.lumen_sgbm_get_schedule(self._h, width, height, ctypes.byref(c))
+        if rc != LUMEN_OK:
+            raise StereoException("invalid size or parameters")
+        return {"agg_prefetch": c.agg_prefetch, "agg_priority": c.agg_priority,
+                "census_tile": c.census_tile}
+
     # Timing getters of include/stereo/sgbm_gpu.h:52-55.
     def getLastComputeTimeMs(self) -> float:
         return float(self._times[0])
@@ -531,3 +565,19 @@ class ServeSGBM:
         st = CServerStats()
         self._lib.lumen_server_get_stats(self._h, ctypes.byref(st))
         return {k: int(getattr(st, k)) for k, _ in CServerStats._fields_}
+
+
+def tuning_load(path: str | None = None) -> int:
+    """Replaces the process-wide tuning table (None: tuning.tsv next to the library). Returns
+    the number of entries; raises StereoException for a malformed file."""
+    n = ctypes.c_int()
+    lib = library()
+    rc = lib.lumen_tuning_load(str(path).encode() if path else None, ctypes.byref(n))
+    if rc != LUMEN_OK:
+        raise StereoException(lib.lumen_last_error().decode())
+    return n.value
+
+
+def tuning_clear() -> None:
+    """Empties the tuning table: every pair runs the default schedule."""
+    library().lumen_tuning_clear()

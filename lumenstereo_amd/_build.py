@@ -5,6 +5,7 @@ compiled in-tree so the shared objects travel with the source tree.
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -19,7 +20,8 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_capi.hip", "sgbm_batch.hip", "sgbm_serve.hip"]
+HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_paths.hip", "sgbm_capi.hip", "sgbm_batch.hip",
+               "sgbm_serve.hip", "sgbm_tune.hip"]
 CXX_SOURCES = ["synth_scene.cpp"]
 
 HIP_FLAGS = [
@@ -42,11 +44,12 @@ def build_library() -> Path:
     inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
     objdir = PKG / "_objs"
     objdir.mkdir(exist_ok=True)
-    objs = []
-    for src in HIP_SOURCES:
-        obj = objdir / (Path(src).stem + ".o")
-        _run([HIPCC, *HIP_FLAGS, *inc, "-c", CSRC / src, "-o", obj])
-        objs.append(obj)
+    objs = [objdir / (Path(src).stem + ".o") for src in HIP_SOURCES]
+    # The translation units are independent: compile them in parallel (bounded by the CPUs).
+    jobs = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
+        list(pool.map(lambda so: _run([HIPCC, *HIP_FLAGS, *inc, "-c", CSRC / so[0], "-o", so[1]]),
+                      zip(HIP_SOURCES, objs)))
     for src in CXX_SOURCES:
         obj = objdir / (Path(src).stem + ".o")
         _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", *inc, "-c", CSRC / src,

@@ -2,8 +2,11 @@
 // mode and path count per stage, in persistent kernels driven by work lists, replayed as one
 // hipGraph.
 //
-// Each stage is one persistent kernel whose blocks (or waves, for the aggregation) take units
-// of work from an atomic counter until the list is exhausted:
+// Each stage is one persistent kernel over a list of units of work. Aggregation waves take
+// their units from an atomic counter (the units differ in length by orders of magnitude and
+// are sorted longest first); the other stages' units are of similar size, so block b takes
+// units b, b + grid, b + 2 grid, ... with no atomics (hundreds of thousands of small units
+// would serialise on one counter):
 //   cost        (pair, row, segment)       block-wide  census or SAD by the pair's mode
 //   paths       (pair, path, 4 scanlines)  wave-wide   list sorted by scanline length, longest
 //                                                      first, so the stage's tail stays short
@@ -30,7 +33,7 @@ using namespace lumen;
 
 namespace {
 
-enum Counter { kCtrCost = 0, kCtrPaths, kCtrPathsBig, kCtrWta, kCtrWtaBig, kCtrSpeckle, kCtrMedian, kNumCounters };
+enum Counter { kCtrPaths = 0, kCtrPathsBig, kNumCounters };
 
 // Device-resident sizes of the current batch (so one built graph serves any batch).
 enum HeaderField { kHdrPairs = 0, kHdrWta0, kHdrWta1, kHdrSpeckle, kHdrItems, kHdrItemsBig, kHdrFields };
@@ -62,15 +65,6 @@ __device__ __forceinline__ int find_pair(const int* __restrict__ start, int npai
     return lo;
 }
 
-// One thread of the block takes the next unit; everybody gets it.
-__device__ __forceinline__ int block_take(int* ctr, int* s_item) {
-    if (threadIdx.x == 0) *s_item = atomicAdd(ctr, 1);
-    __syncthreads();
-    const int it = *s_item;
-    __syncthreads();
-    return it;
-}
-
 __global__ void reset_counters_kernel(int* ctr) {
     if (threadIdx.x < kNumCounters) ctr[threadIdx.x] = 0;
 }
@@ -78,15 +72,12 @@ __global__ void reset_counters_kernel(int* ctr) {
 // ----------------------------------------------------------------------------------- cost
 // Units: census pairs contribute ceil(W/256)*H segments, SAD pairs ceil(W/64)*H.
 __global__ __launch_bounds__(256) void batch_cost_kernel(const PairJob* __restrict__ jobs,
-                                                         Tables t, int* ctr) {
+                                                         Tables t) {
     extern __shared__ uint32_t lds[];
-    __shared__ int s_item;
     const int npairs = t.hdr[kHdrPairs];
     const int* start = t.costStart;
     const int total = start[npairs];
-    for (;;) {
-        const int u = block_take(&ctr[kCtrCost], &s_item);
-        if (u >= total) break;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
         const int p = find_pair(start, npairs, u);
         const PairJob j = jobs[p];
         const int local = u - start[p];
@@ -142,16 +133,13 @@ __global__ __launch_bounds__(256) void batch_paths_kernel(const PairJob* __restr
 // Units: rows of the pairs in this K set (prefix sums over those pairs only).
 template <int KSET>
 __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restrict__ jobs,
-                                                        Tables t, int* ctr) {
+                                                        Tables t) {
     extern __shared__ unsigned long long rbest[];
-    __shared__ int s_item;
     const int npairs = t.hdr[KSET == 0 ? kHdrWta0 : kHdrWta1];
     const int* start = KSET == 0 ? t.wta0Start : t.wta1Start;
     const int* pairIdx = KSET == 0 ? t.wta0Pairs : t.wta1Pairs;
     const int total = start[npairs];
-    for (;;) {
-        const int u = block_take(&ctr[KSET == 0 ? kCtrWta : kCtrWtaBig], &s_item);
-        if (u >= total) break;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
         const int q = find_pair(start, npairs, u);
         const PairJob j = jobs[pairIdx[q]];
         const int y = u - start[q];
@@ -176,15 +164,12 @@ __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restric
 
 // ----------------------------------------------------------------------------------- post
 __global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __restrict__ jobs,
-                                                            Tables t, int* ctr) {
-    __shared__ int s_item;
+                                                            Tables t) {
     const int npairs = t.hdr[kHdrSpeckle];
     const int* start = t.spkStart;
     const int* pairIdx = t.spkPairs;
     const int total = start[npairs];
-    for (;;) {
-        const int u = block_take(&ctr[kCtrSpeckle], &s_item);
-        if (u >= total) break;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
         const int q = find_pair(start, npairs, u);
         const PairJob j = jobs[pairIdx[q]];
         const int tiles = (j.W + 15) / 16;
@@ -197,15 +182,12 @@ __global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __res
 }
 
 __global__ __launch_bounds__(256) void batch_median_kernel(const PairJob* __restrict__ jobs,
-                                                           Tables t, int* ctr) {
+                                                           Tables t) {
     __shared__ int16_t tile[kMedTH + 2][kMedTW + 2];
-    __shared__ int s_item;
     const int npairs = t.hdr[kHdrPairs];
     const int* start = t.medStart;
     const int total = start[npairs];
-    for (;;) {
-        const int u = block_take(&ctr[kCtrMedian], &s_item);
-        if (u >= total) break;
+    for (int u = blockIdx.x; u < total; u += gridDim.x) {
         const int p = find_pair(start, npairs, u);
         const PairJob j = jobs[p];
         const int tiles = (j.W + kMedTW - 1) / kMedTW;
@@ -438,7 +420,7 @@ int BatchEngine::ensure_graph(const BatchPlan& pl, std::string& err) {
         prev = node;
     };
     void* aReset[] = {&ctr_};
-    void* aStage[] = {&jobs_, &t, &ctr_};
+    void* aStage[] = {&jobs_, &t};
     void* aPaths[] = {&jobs_, &items_, &t, &ctr_, &trash_};
     void* aPathsBig[] = {&jobs_, &itemsBig_, &t, &ctr_, &trash_};
     add(reinterpret_cast<const void*>(&reset_counters_kernel), 1, 64, 0, aReset);

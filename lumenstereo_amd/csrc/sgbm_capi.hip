@@ -108,6 +108,8 @@ struct lumen_sgbm {
     DevBuf left, right, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
         sum, trash;
     PinnedBuf stageIn, stageOut;
+    bool hasSchedule = false;  // lumen_sgbm_set_schedule override
+    Schedule schedule{};
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -152,7 +154,13 @@ namespace {
 // Validation of src/sgbm_gpu.cu:148-166, plus the limits of this implementation.
 int validate(lumen_sgbm* h, const void* left, const void* right, int W, int H) {
     const std::string msg = validate_pair(h->params, W, H, left != nullptr && right != nullptr);
-    return msg.empty() ? LUMEN_OK : h->fail(LUMEN_EINVAL, msg);
+    if (!msg.empty()) return h->fail(LUMEN_EINVAL, msg);
+    if (h->hasSchedule) {
+        std::string why;
+        const int K = padded_range(h->params.maxDisparity - h->params.minDisparity) / kScanLanes;
+        if (!valid_schedule(h->schedule, K, why)) return h->fail(LUMEN_EINVAL, "schedule: " + why);
+    }
+    return LUMEN_OK;
 }
 
 int prepare(lumen_sgbm* h, int W, int H) {
@@ -200,6 +208,7 @@ int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dou
     hipStream_t s = h->stream;
     PairJob j{};
     fill_job(j, h->params, g.W, g.H);
+    if (h->hasSchedule) j.sched = h->schedule;  // checked by validate()
     j.left = dl;
     j.right = dr;
     j.cost = h->cost.as<uint8_t>();
@@ -258,6 +267,8 @@ int check_params(const lumen_stereo_params* p) {
 }
 
 }  // namespace
+
+void lumen::set_last_error(const std::string& msg) { g_last_error = msg; }
 
 extern "C" {
 
@@ -320,6 +331,31 @@ int lumen_sgbm_get_params(const lumen_sgbm* h, lumen_stereo_params* out) {
 }
 
 int lumen_sgbm_num_paths(const lumen_sgbm* h) { return h ? num_paths(h->params) : 0; }
+
+int lumen_sgbm_set_schedule(lumen_sgbm* h, const lumen_schedule* s) {
+    if (!h) return LUMEN_EINVAL;
+    if (!s) {
+        h->hasSchedule = false;
+        return LUMEN_OK;
+    }
+    h->schedule = Schedule{s->agg_prefetch, s->agg_priority, s->census_tile};
+    h->hasSchedule = true;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_get_schedule(const lumen_sgbm* h, int width, int height, lumen_schedule* out) {
+    if (!h || !out || width <= 0 || height <= 0) return LUMEN_EINVAL;
+    const int D = h->params.maxDisparity - h->params.minDisparity;
+    if (D < 1 || D > LUMEN_MAX_DISPARITY_RANGE) return LUMEN_EINVAL;
+    const Schedule s = h->hasSchedule ? h->schedule
+                                      : tuned_schedule(width, padded_range(D), h->params.matchingCostMode,
+                                                       num_paths(h->params));
+    out->agg_prefetch = s.aggPrefetch;
+    out->agg_priority = s.aggPriority;
+    out->census_tile = s.censusTile;
+    out->reserved = 0;
+    return LUMEN_OK;
+}
 
 int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int height,
                          size_t* bytes_out) {

@@ -114,12 +114,14 @@ __device__ __forceinline__ float parabola(float s, uint32_t a0, uint32_t a1, uin
 // neighbour is >= the centre; code 0 within 2 px of the border), then each thread writes 16
 // consecutive disparities of one pixel as one 16-byte store.
 // ---------------------------------------------------------------------------------------
-constexpr int kCensusTile = 256;  // pixels per census cost segment
+constexpr int kCensusTile = 256;  // default pixels per census cost segment (batched form)
 constexpr int kSadTile = 64;      // pixels per SAD cost segment
 
-__host__ __device__ inline size_t census_lds_bytes(int Dp) {
-    const int nr = kCensusTile + Dp - 1;
-    return (size_t)(kCensusTile + nr) * 4 + 5 * (kCensusTile + 4) + 5 * (nr + 4);
+// LDS of a census segment of T pixels: left codes, the codes of every right pixel the segment
+// can reach, and the five staged rows of both images.
+__host__ __device__ inline size_t census_lds_bytes(int Dp, int T = kCensusTile) {
+    const int nr = T + Dp - 1;
+    return (size_t)(T + nr) * 4 + 5 * (T + 4) + 5 * (nr + 4);
 }
 
 // SAD segment LDS: prefiltered left/right tiles stored column-major (the 2r+1 rows of a column
@@ -147,9 +149,10 @@ __device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, in
     return code;
 }
 
-// Block-wide; every thread of the block must call it with the same (y, x0).
+// Block-wide; every thread of the block must call it with the same (y, x0). T is the segment
+// width (a schedule choice; the costs do not depend on it).
+template <int T = kCensusTile>
 __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* shc) {
-    constexpr int T = kCensusTile;
     const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int rs = x0 - m - (Dp - 1);  // first right pixel the segment can reach
@@ -512,6 +515,13 @@ __device__ __forceinline__ void store_path(uint16_t* p, const u16x2 (&v)[K / 2])
             __builtin_nontemporal_store(
                 u32x4{bits(v[4 * i]), bits(v[4 * i + 1]), bits(v[4 * i + 2]), bits(v[4 * i + 3])},
                 reinterpret_cast<u32x4*>(p) + i);
+    } else if constexpr (K >= 20) {
+        // 8-byte pieces at a stride of 2K >= 40 bytes: one store instruction covers a row's
+        // 32K bytes only sparsely, so the pieces are left in L2 to merge into whole lines
+        // (nontemporal pieces reach HBM as partial writes: 28 % more write traffic at K = 20).
+#pragma unroll
+        for (int i = 0; i < K / 4; ++i)
+            reinterpret_cast<u32x2*>(p)[i] = u32x2{bits(v[2 * i]), bits(v[2 * i + 1])};
     } else {
 #pragma unroll
         for (int i = 0; i < K / 4; ++i)
@@ -583,8 +593,10 @@ __device__ __forceinline__ uint32_t p2_lookup(const P2Table& t, uint32_t g) {
 
 // Scanlines s0 .. s0+3 of `path` of pair j; wave-wide (no block synchronisation). Lanes of a
 // DPP row whose scanline does not exist compute on a copy of pixel 0 and store to `trash`.
-template <int K, bool PAD>
+// PF is the prefetch depth (a schedule choice; the results do not depend on it).
+template <int K, bool PAD, int PF = default_prefetch(K)>
 __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_base) {
+    static_assert(valid_prefetch(K, PF), "unsupported prefetch depth for this slice");
     const int lane = threadIdx.x & 63;
     const int g = lane >> 4, l = lane & 15;
     const int s = s0 + g;
@@ -633,7 +645,6 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     // value with DPP row_newbcast. Loads past the end of a scanline re-read its last pixel;
     // steps past the end are computed but stored to the trash slot, so every store is
     // unconditional and the wait counts stay exact.
-    constexpr int PF = K <= 8 ? 8 : 4;
     const int last = len > 0 ? len - 1 : 0;
     const int lu = l & (PF - 1);
     auto fetch = [&](int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
@@ -697,7 +708,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
         run(t0 + PF, bufB, gB);
         // Block A's loads were issued before block B's stores: waiting until only those
         // stores may be outstanding retires the loads without waiting on the stores.
-        constexpr int N = PF * (K % 8 == 0 ? K / 8 : K / 4);
+        constexpr int N = PF * path_stores(K);
         static_assert(N < 64, "vmcnt is 6 bits");
         LUMEN_WAIT_VMCNT(N);
     }

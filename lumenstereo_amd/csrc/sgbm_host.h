@@ -19,7 +19,16 @@ inline int cost_scale(const lumen_stereo_params& p) {
     return std::max(1, (p.blockSize * p.blockSize * 2 * p.preFilterCap) / 255);
 }
 
-// Scalar fields of a PairJob from the parameters and image size (pointers untouched).
+// Schedules (sgbm_tune.hip): whether a schedule is valid for slice K (the reason in `why` when
+// it is not), and the tuned schedule of a pair (the default when the table has no entry).
+bool valid_schedule(const Schedule& s, int K, std::string& why);
+Schedule tuned_schedule(int W, int Dp, int mode, int npaths);
+
+// Sets the text lumen_last_error() returns on this thread (sgbm_capi.hip).
+void set_last_error(const std::string& msg);
+
+// Scalar fields of a PairJob from the parameters and image size (pointers untouched), with
+// the tuned schedule.
 inline void fill_job(PairJob& j, const lumen_stereo_params& p, int W, int H) {
     j.W = W;
     j.H = H;
@@ -41,6 +50,7 @@ inline void fill_job(PairJob& j, const lumen_stereo_params& p, int W, int H) {
     j.doSpeckle = p.speckleWindowSize > 0 ? 1 : 0;
     j.speckleMin = std::min(p.speckleWindowSize, 25);
     j.speckleDiff = p.speckleRange * 16;
+    j.sched = tuned_schedule(W, j.Dp, j.mode, j.npaths);
 }
 
 // Device bytes of one pair's working set: images, padded cost volume, per-path uint16

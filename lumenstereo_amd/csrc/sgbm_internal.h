@@ -19,6 +19,28 @@ constexpr int kTrashElems = 64 * 32;
 // Disparity-innermost layouts are padded to a multiple of 64 entries (16 lanes x 4).
 inline int padded_range(int D) { return (D + 63) / 64 * 64; }
 
+// Vector stores per aggregation step and lane: K/8 16-byte stores, or K/4 8-byte ones.
+__host__ __device__ constexpr int path_stores(int K) { return K % 8 == 0 ? K / 8 : K / 4; }
+
+// Prefetch depths the aggregation supports for slice K: each lane of a DPP row fetches the
+// image bytes of at most one step of a block (PF <= 16), and the stores of one block must fit
+// the 6-bit vmcnt the loop waits on (PF * stores < 64).
+__host__ __device__ constexpr bool valid_prefetch(int K, int PF) {
+    return (PF == 4 || PF == 8 || PF == 16) && PF * path_stores(K) < 64;
+}
+
+__host__ __device__ constexpr int default_prefetch(int K) { return K <= 8 ? 8 : 4; }
+
+// Schedule of one pair: how its work is grouped and ordered, never what is computed. Every
+// valid schedule gives bit-identical results.
+struct Schedule {
+    int aggPrefetch;  // aggregation cost prefetch depth: 4, 8 or 16 (valid_prefetch)
+    int aggPriority;  // aggregation wave priority: 0 none, 1 by path kind, 2 by scanline length
+    int censusTile;   // census cost segment width: 128, 256 or 512 pixels
+};
+
+inline Schedule default_schedule(int K) { return Schedule{default_prefetch(K), 1, 256}; }
+
 // Everything the device stages need to know about one stereo pair.
 struct PairJob {
     const uint8_t* left;   // [H][W] grayscale
@@ -39,6 +61,7 @@ struct PairJob {
     int uniquenessRatio;
     int doLR, lrMaxDiff;   // 16 * disp12MaxDiff
     int doSpeckle, speckleMin, speckleDiff;
+    Schedule sched;
 };
 
 // Per-pair launches (sgbm_kernels.hip).

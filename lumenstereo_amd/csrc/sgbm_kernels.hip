@@ -1,4 +1,5 @@
-// sgbm_kernels.hip - per-pair kernels of the matcher (one launch per stage for one pair).
+// sgbm_kernels.hip - per-pair kernels of the matcher (one launch per stage for one pair):
+// cost, sum/WTA/LR, post-processing. The aggregation kernels are in sgbm_paths.hip.
 //
 // The stage bodies live in sgbm_device.h and are shared with the persistent batched kernels
 // (sgbm_batch.hip); these kernels map one grid onto one pair's units of work.
@@ -6,12 +7,18 @@
 
 namespace lumen {
 
-int scanline_count(int path, int W, int H) { return scan_count(path, W, H); }
-
 // ------------------------------------------------------------------------------- cost
+template <int T>
 __global__ __launch_bounds__(256) void cost_census_kernel(PairJob j) {
     extern __shared__ uint32_t shc[];
-    census_cost_segment(j, blockIdx.y, blockIdx.x * kCensusTile, shc);
+    census_cost_segment<T>(j, blockIdx.y, blockIdx.x * T, shc);
+}
+
+template <int T>
+static hipError_t launch_census(const PairJob& j, hipStream_t s) {
+    dim3 grid((j.W + T - 1) / T, j.H);
+    hipLaunchKernelGGL(cost_census_kernel<T>, grid, dim3(256), census_lds_bytes(j.Dp, T), s, j);
+    return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void cost_sad_kernel(PairJob j) {
@@ -32,9 +39,12 @@ __global__ __launch_bounds__(256) void cost_sad_direct_kernel(PairJob j) {
 
 hipError_t launch_cost(const PairJob& j, hipStream_t s) {
     if (j.mode == 1) {
-        dim3 grid((j.W + kCensusTile - 1) / kCensusTile, j.H);
-        hipLaunchKernelGGL(cost_census_kernel, grid, dim3(256), census_lds_bytes(j.Dp), s, j);
-        return hipGetLastError();
+        switch (j.sched.censusTile) {
+            case 128: return launch_census<128>(j, s);
+            case 256: return launch_census<256>(j, s);
+            case 512: return launch_census<512>(j, s);
+            default: return hipErrorInvalidValue;
+        }
     }
     const int r = j.blockSize / 2;
     if (r < 0) {
@@ -50,66 +60,6 @@ hipError_t launch_cost(const PairJob& j, hipStream_t s) {
                            0, s, j);
     }
     return hipGetLastError();
-}
-
-// ------------------------------------------------------------------------------- paths
-// All paths of the pair in one launch: wave w takes scanlines 4w'..4w'+3 of the path whose
-// range of waves contains w (paths in order, so the long horizontal scanlines start first).
-struct PathsGrid {
-    int waveStart[kMaxPaths + 1];
-};
-
-template <int K, bool PAD>
-__global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, uint16_t* trash) {
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= g.waveStart[j.npaths]) return;
-    int path = 0;
-#pragma unroll
-    for (int r = 1; r < kMaxPaths; ++r)
-        if (r < j.npaths && wave >= g.waveStart[r]) path = r;
-    const int s0 = (wave - g.waveStart[path]) * 4;
-    // The launch ends when its longest scanlines end: the longer straight paths get the higher
-    // issue priority within their SIMD, diagonals the lowest.
-    const bool longest = j.W >= j.H ? path < 2 : (path >= 2 && path < 4);
-    if (longest)
-        __builtin_amdgcn_s_setprio(3);
-    else if (path < 4)
-        __builtin_amdgcn_s_setprio(2);
-    else
-        __builtin_amdgcn_s_setprio(1);
-    paths_wave<K, PAD>(j, path, s0, trash);
-}
-
-template <int K>
-static hipError_t launch_paths_k(const PairJob& j, const PathsGrid& g, int blocks,
-                                 uint16_t* trash, hipStream_t s) {
-    if (j.D == j.Dp)
-        hipLaunchKernelGGL((sgm_paths_kernel<K, false>), dim3(blocks), dim3(256), 0, s, j, g, trash);
-    else
-        hipLaunchKernelGGL((sgm_paths_kernel<K, true>), dim3(blocks), dim3(256), 0, s, j, g, trash);
-    return hipGetLastError();
-}
-
-hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s) {
-    PathsGrid g;
-    int waves = 0;
-    for (int r = 0; r < j.npaths; ++r) {
-        g.waveStart[r] = waves;
-        waves += (scan_count(r, j.W, j.H) + 3) / 4;
-    }
-    for (int r = j.npaths; r <= kMaxPaths; ++r) g.waveStart[r] = waves;
-    const int blocks = (waves + 3) / 4;
-    switch (j.K) {
-        case 4: return launch_paths_k<4>(j, g, blocks, trash, s);
-        case 8: return launch_paths_k<8>(j, g, blocks, trash, s);
-        case 12: return launch_paths_k<12>(j, g, blocks, trash, s);
-        case 16: return launch_paths_k<16>(j, g, blocks, trash, s);
-        case 20: return launch_paths_k<20>(j, g, blocks, trash, s);
-        case 24: return launch_paths_k<24>(j, g, blocks, trash, s);
-        case 28: return launch_paths_k<28>(j, g, blocks, trash, s);
-        case 32: return launch_paths_k<32>(j, g, blocks, trash, s);
-        default: return hipErrorInvalidValue;
-    }
 }
 
 // ------------------------------------------------------------------------------- WTA

@@ -1,0 +1,91 @@
+// sgbm_paths.hip - per-pair SGM aggregation launch: all paths of a pair in one grid.
+//
+// One instantiation per (slice K, padding, prefetch depth): the prefetch depth is a schedule
+// choice (Schedule::aggPrefetch), so each pair runs the instantiation its schedule names.
+#include "sgbm_device.h"
+
+namespace lumen {
+
+int scanline_count(int path, int W, int H) { return scan_count(path, W, H); }
+
+// Wave w takes scanlines 4w'..4w'+3 of the path whose range of waves contains w (paths in
+// order, so the long horizontal scanlines start first).
+struct PathsGrid {
+    int waveStart[kMaxPaths + 1];
+};
+
+template <int K, bool PAD, int PF>
+__global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, uint16_t* trash) {
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= g.waveStart[j.npaths]) return;
+    int path = 0;
+#pragma unroll
+    for (int r = 1; r < kMaxPaths; ++r)
+        if (r < j.npaths && wave >= g.waveStart[r]) path = r;
+    const int s0 = (wave - g.waveStart[path]) * 4;
+    // The launch ends when its longest scanlines end. Priority 1: the longer straight path
+    // first, then the other straight path, diagonals last; priority 2: by scanline length.
+    if (j.sched.aggPriority == 1) {
+        const bool longest = j.W >= j.H ? path < 2 : (path >= 2 && path < 4);
+        if (longest)
+            __builtin_amdgcn_s_setprio(3);
+        else if (path < 4)
+            __builtin_amdgcn_s_setprio(2);
+        else
+            __builtin_amdgcn_s_setprio(1);
+    } else if (j.sched.aggPriority == 2) {
+        set_priority_for_length(scan_length(path, s0, j.W, j.H));
+    }
+    paths_wave<K, PAD, PF>(j, path, s0, trash);
+}
+
+template <int K, int PF>
+static hipError_t launch_kp(const PairJob& j, const PathsGrid& g, int blocks, uint16_t* trash,
+                            hipStream_t s) {
+    if constexpr (!valid_prefetch(K, PF)) {
+        return hipErrorInvalidValue;
+    } else {
+        if (j.D == j.Dp)
+            hipLaunchKernelGGL((sgm_paths_kernel<K, false, PF>), dim3(blocks), dim3(256), 0, s, j,
+                               g, trash);
+        else
+            hipLaunchKernelGGL((sgm_paths_kernel<K, true, PF>), dim3(blocks), dim3(256), 0, s, j,
+                               g, trash);
+        return hipGetLastError();
+    }
+}
+
+template <int K>
+static hipError_t launch_k(const PairJob& j, const PathsGrid& g, int blocks, uint16_t* trash,
+                           hipStream_t s) {
+    switch (j.sched.aggPrefetch) {
+        case 4: return launch_kp<K, 4>(j, g, blocks, trash, s);
+        case 8: return launch_kp<K, 8>(j, g, blocks, trash, s);
+        case 16: return launch_kp<K, 16>(j, g, blocks, trash, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s) {
+    PathsGrid g;
+    int waves = 0;
+    for (int r = 0; r < j.npaths; ++r) {
+        g.waveStart[r] = waves;
+        waves += (scan_count(r, j.W, j.H) + 3) / 4;
+    }
+    for (int r = j.npaths; r <= kMaxPaths; ++r) g.waveStart[r] = waves;
+    const int blocks = (waves + 3) / 4;
+    switch (j.K) {
+        case 4: return launch_k<4>(j, g, blocks, trash, s);
+        case 8: return launch_k<8>(j, g, blocks, trash, s);
+        case 12: return launch_k<12>(j, g, blocks, trash, s);
+        case 16: return launch_k<16>(j, g, blocks, trash, s);
+        case 20: return launch_k<20>(j, g, blocks, trash, s);
+        case 24: return launch_k<24>(j, g, blocks, trash, s);
+        case 28: return launch_k<28>(j, g, blocks, trash, s);
+        case 32: return launch_k<32>(j, g, blocks, trash, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lumen

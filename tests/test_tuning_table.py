@@ -1,0 +1,95 @@
+"""The tuned-schedule table (host side, no GPU work): parsing, validation with the library's
+own rules, bucket lookup by width, and the defaults. The table only chooses schedules, which
+never change results (tests/test_gpu_tune.py checks that on the GPU)."""
+import pytest
+
+import lumenstereo_amd as ls
+from lumenstereo_amd import tune
+
+TABLE = ls.LIB_PATH.parent / "tuning.tsv"
+
+
+@pytest.fixture(autouse=True)
+def restore_table():
+    yield
+    if TABLE.exists():
+        ls.tuning_load(str(TABLE))
+    else:
+        ls.tuning_clear()
+
+
+def _handle(D, census, paths):
+    return ls.StereoSGBM(tune.params_for(D, census, paths))
+
+
+@pytest.mark.parametrize("D,K", [(64, 4), (128, 8), (192, 12), (290, 20), (512, 32)])
+def test_defaults_without_table(D, K):
+    ls.tuning_clear()
+    s = _handle(D, True, 8).getSchedule(640, 480)
+    assert s == tune.default_schedule(K)
+
+
+def test_bucket_lookup(tmp_path):
+    f = tmp_path / "t.tsv"
+    f.write_text("# comment\n\n640 128 1 8 16 2 128   # trailing comment\n"
+                 "1920 128 1 8 4 0 512\n640 128 0 8 4 2 256\n")
+    assert ls.tuning_load(str(f)) == 3
+    m = _handle(128, True, 8)
+    assert m.getSchedule(320, 240)["agg_prefetch"] == 16      # smallest bucket >= width
+    assert m.getSchedule(640, 480)["census_tile"] == 128
+    assert m.getSchedule(641, 480)["census_tile"] == 512      # next bucket up
+    assert m.getSchedule(4000, 100)["agg_priority"] == 0      # wider than every bucket
+    assert _handle(100, True, 8).getSchedule(640, 480)["agg_prefetch"] == 16  # D=100 pads to 128
+    assert _handle(128, False, 8).getSchedule(640, 480)["agg_prefetch"] == 4  # SAD entry
+    assert _handle(128, True, 4).getSchedule(640, 480) == tune.default_schedule(8)  # no entry
+
+
+def test_override_beats_table(tmp_path):
+    f = tmp_path / "t.tsv"
+    f.write_text("640 128 1 8 16 2 128\n")
+    ls.tuning_load(str(f))
+    m = _handle(128, True, 8)
+    m.setSchedule({"agg_prefetch": 4, "agg_priority": 0, "census_tile": 512})
+    assert m.getSchedule(640, 480) == {"agg_prefetch": 4, "agg_priority": 0, "census_tile": 512}
+    m.setSchedule(None)
+    assert m.getSchedule(640, 480)["agg_prefetch"] == 16
+
+
+@pytest.mark.parametrize("line,msg", [
+    ("640 128 1 8 16 2", "expected 7 integers"),
+    ("640 100 1 8 8 1 256", "bad width or padded range"),
+    ("640 128 2 8 8 1 256", "bad cost mode"),
+    ("640 128 1 6 8 1 256", "bad cost mode or path count"),
+    ("640 128 1 8 8 3 256", "priority"),
+    ("640 128 1 8 8 1 300", "census tile"),
+    ("640 320 1 4 16 1 256", "not supported"),
+])
+def test_malformed_tables_are_rejected(tmp_path, line, msg):
+    f = tmp_path / "t.tsv"
+    f.write_text("640 64 0 4 8 1 256\n" + line + "\n")
+    ls.tuning_clear()
+    with pytest.raises(ls.StereoException, match=msg):
+        ls.tuning_load(str(f))
+    # the table is unchanged by a failed load
+    assert _handle(64, False, 4).getSchedule(640, 480) == tune.default_schedule(4)
+
+
+def test_prefetch_rule_matches_library(tmp_path):
+    """The tuner's candidate filter agrees with the library's validation for every slice."""
+    for K in range(4, 33, 4):
+        for pf in (4, 8, 16):
+            f = tmp_path / f"k{K}_{pf}.tsv"
+            f.write_text(f"640 {K * 16} 1 8 {pf} 1 256\n")
+            try:
+                ls.tuning_load(str(f))
+                ok = True
+            except ls.StereoException:
+                ok = False
+            assert ok == tune.valid_prefetch(K, pf), (K, pf)
+    for K in range(4, 33, 4):
+        assert tune.valid_prefetch(K, tune.default_schedule(K)["agg_prefetch"])
+
+
+def test_missing_file_is_an_error():
+    with pytest.raises(ls.StereoException, match="cannot open"):
+        ls.tuning_load("/nonexistent/tuning.tsv")
