@@ -201,8 +201,8 @@ const char* lumen_batch_last_error(const lumen_batch* b);
  * and path count) are submitted as they arrive and return a ticket. A dispatcher thread admits
  * them into a device pool by their working-set size (cost volume, path buffers, maps), and
  * whenever one of the lanes (a stream with its own persistent-kernel pipeline) is free it starts
- * every admitted request that has not started as one batch on it. So requests that arrive
- * while others are on the GPU run alongside them. Every result is bit-identical to
+ * admitted requests of similar size as one batch on it. So requests that arrive while others
+ * are on the GPU run alongside them. Every result is bit-identical to
  * lumen_sgbm_compute of the same pair with the same parameters.
  * ---------------------------------------------------------------------------------------- */
 typedef struct lumen_server_config {
@@ -211,6 +211,11 @@ typedef struct lumen_server_config {
     int32_t max_batch;   /* most requests one batch takes; 0 = 64 */
     int32_t device_io;   /* 0: request images/outputs are host memory; 1: device memory */
     int32_t max_bypass;  /* smaller requests that may pass a waiting one; 0 = 8, -1 = none */
+    int32_t batch_budget_us; /* work one batch may hold, in microseconds at 5 TB/s of its
+                                algorithmic HBM traffic (W*H*Dp*(1+5*paths) bytes); 0 = 2000.
+                                A batch only groups requests within 4x of each other's work, so
+                                small requests do not wait for a large one. */
+    int32_t reserved;
 } lumen_server_config;
 
 typedef struct lumen_request_info {

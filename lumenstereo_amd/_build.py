@@ -22,7 +22,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_paths.hip", "sgbm_capi.hip", "sgbm_batch.hip",
                "sgbm_serve.hip", "sgbm_tune.hip"]
-CXX_SOURCES = ["synth_scene.cpp"]
+CXX_SOURCES = ["synth_scene.cpp", "host_copy.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950",
@@ -52,8 +52,8 @@ def build_library() -> Path:
                       zip(HIP_SOURCES, objs)))
     for src in CXX_SOURCES:
         obj = objdir / (Path(src).stem + ".o")
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", *inc, "-c", CSRC / src,
-              "-o", obj])
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-pthread", *inc, "-c",
+              CSRC / src, "-o", obj])
         objs.append(obj)
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-pthread", "-o", tmp])
@@ -89,7 +89,8 @@ SERVE_TEST_BIN = ROOT / "tests" / "cpp" / "test_serve_core"
 def build_serve_core_test() -> Path:
     """Host-only test program of the serving loop's pool and admission logic (no HIP)."""
     tmp = SERVE_TEST_BIN.with_suffix(".tmp")
-    _run(["g++", "-O1", "-std=c++17", "-I", CSRC, SERVE_TEST_SRC, "-o", tmp])
+    _run(["g++", "-O1", "-std=c++17", "-pthread", "-I", CSRC, SERVE_TEST_SRC, CSRC / "host_copy.cpp",
+          "-o", tmp])
     os.replace(tmp, SERVE_TEST_BIN)
     return SERVE_TEST_BIN
 

@@ -214,6 +214,30 @@ private:
     std::deque<Entry> queue_;
 };
 
+// Which ready requests (indices into `work`, in queue order) the next batch takes. A batch
+// returns when its last pair finishes, so a small request batched with a large one waits for
+// the large one: the batch starts with the oldest ready request and adds later ones only when
+// their estimated work is within a factor `spread` of the first's either way, while the total
+// stays within max(budget, first's work) and the count within maxBatch. Requests left out
+// stay ready, in order, for the next free lane.
+inline std::vector<size_t> pick_batch(const std::vector<double>& work, size_t maxBatch,
+                                      double budget, double spread) {
+    std::vector<size_t> out;
+    if (work.empty() || maxBatch == 0) return out;
+    const double first = work[0];
+    const double cap = std::max(budget, first);
+    double total = first;
+    out.push_back(0);
+    for (size_t i = 1; i < work.size() && out.size() < maxBatch; ++i) {
+        const double w = work[i];
+        if (w > first * spread || w * spread < first) continue;
+        if (total + w > cap) continue;
+        total += w;
+        out.push_back(i);
+    }
+    return out;
+}
+
 // Nearest-rank percentile (q in [0, 100]) of a sample; 0 for an empty one.
 inline double percentile(std::vector<double> v, double q) {
     if (v.empty()) return 0.0;

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "batch_engine.h"
+#include "host_copy.h"
 #include "lumen_sgbm.h"
 #include "sgbm_device.h"
 #include "sgbm_host.h"
@@ -65,6 +66,21 @@ __device__ __forceinline__ int find_pair(const int* __restrict__ start, int npai
     return lo;
 }
 
+// The pair of unit u, kept across the units of one block: with units b, b + grid, ... a block
+// mostly stays inside one pair, so the binary search and the PairJob load (a chain of
+// dependent global loads that costs more than a small unit's work) happen once per pair.
+struct PairCursor {
+    int idx = -1, beg = 0, end = 0;
+    PairJob job;
+    __device__ __forceinline__ bool seek(const int* __restrict__ start, int npairs, int u) {
+        if (u >= beg && u < end) return false;
+        idx = find_pair(start, npairs, u);
+        beg = start[idx];
+        end = start[idx + 1];
+        return true;
+    }
+};
+
 __global__ void reset_counters_kernel(int* ctr) {
     if (threadIdx.x < kNumCounters) ctr[threadIdx.x] = 0;
 }
@@ -77,10 +93,11 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const PairJob* __restri
     const int npairs = t.hdr[kHdrPairs];
     const int* start = t.costStart;
     const int total = start[npairs];
+    PairCursor c;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int p = find_pair(start, npairs, u);
-        const PairJob j = jobs[p];
-        const int local = u - start[p];
+        if (c.seek(start, npairs, u)) c.job = jobs[c.idx];
+        const PairJob& j = c.job;
+        const int local = u - c.beg;
         if (j.mode == 1) {
             const int segs = (j.W + kCensusTile - 1) / kCensusTile;
             const int y = local / segs;
@@ -139,10 +156,11 @@ __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restric
     const int* start = KSET == 0 ? t.wta0Start : t.wta1Start;
     const int* pairIdx = KSET == 0 ? t.wta0Pairs : t.wta1Pairs;
     const int total = start[npairs];
+    PairCursor c;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int q = find_pair(start, npairs, u);
-        const PairJob j = jobs[pairIdx[q]];
-        const int y = u - start[q];
+        if (c.seek(start, npairs, u)) c.job = jobs[pairIdx[c.idx]];
+        const PairJob& j = c.job;
+        const int y = u - c.beg;
         if constexpr (KSET == 0) {
             switch (j.K) {
                 case 4: wta_row<4, true>(j, y, rbest); break;
@@ -169,11 +187,12 @@ __global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __res
     const int* start = t.spkStart;
     const int* pairIdx = t.spkPairs;
     const int total = start[npairs];
+    PairCursor c;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int q = find_pair(start, npairs, u);
-        const PairJob j = jobs[pairIdx[q]];
+        if (c.seek(start, npairs, u)) c.job = jobs[pairIdx[c.idx]];
+        const PairJob& j = c.job;
         const int tiles = (j.W + 15) / 16;
-        const int local = u - start[q];
+        const int local = u - c.beg;
         const int ty = local / tiles, tx = local - ty * tiles;
         const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
         if (x < j.W && y < j.H)
@@ -187,11 +206,12 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const PairJob* __rest
     const int npairs = t.hdr[kHdrPairs];
     const int* start = t.medStart;
     const int total = start[npairs];
+    PairCursor c;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        const int p = find_pair(start, npairs, u);
-        const PairJob j = jobs[p];
+        if (c.seek(start, npairs, u)) c.job = jobs[c.idx];
+        const PairJob& j = c.job;
         const int tiles = (j.W + kMedTW - 1) / kMedTW;
-        const int local = u - start[p];
+        const int local = u - c.beg;
         const int ty = local / tiles, tx = local - ty * tiles;
         median_tile(j.doSpeckle ? j.spk : j.dispLR, j.out, j.W, j.H, tx * kMedTW, ty * kMedTH, tile);
         __syncthreads();
@@ -667,12 +687,14 @@ int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs, in
             if (e != hipSuccess) return b->hip_fail(e, "pinned staging");
             e = hipStreamSynchronize(s);  // staging is reused between sub-batches
             if (e != hipSuccess) return b->hip_fail(e, "sync");
+            std::vector<CopyJob> stage;
             for (int i = 0; i < cnt; ++i) {
                 const lumen_pair_desc& d = pairs[first + i];
                 const size_t n = (size_t)d.width * d.height;
-                std::memcpy(b->hostIn + inOff[i], d.left, n);
-                std::memcpy(b->hostIn + inOff[i] + n, d.right, n);
+                stage.push_back({b->hostIn + inOff[i], d.left, n});
+                stage.push_back({b->hostIn + inOff[i] + n, d.right, n});
             }
+            parallel_copy(stage);
             e = hipEventRecord(b->ev[1], s);
             for (int i = 0; e == hipSuccess && i < cnt; ++i) {
                 const size_t n = (size_t)pl.jobs[i].W * pl.jobs[i].H;
@@ -701,10 +723,12 @@ int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs, in
             }
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) return b->hip_fail(e, "download");
+            std::vector<CopyJob> outs;
             for (int i = 0; i < cnt; ++i) {
                 const lumen_pair_desc& d = pairs[first + i];
-                std::memcpy(d.out, b->hostOut + outOff[i] / 2, (size_t)d.width * d.height * 2);
+                outs.push_back({d.out, b->hostOut + outOff[i] / 2, (size_t)d.width * d.height * 2});
             }
+            parallel_copy(outs);
         } else {
             e = hipStreamSynchronize(s);
             if (e != hipSuccess) return b->hip_fail(e, "sync");

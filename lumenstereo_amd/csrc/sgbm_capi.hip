@@ -6,6 +6,7 @@
 // Buffers grow on demand and are kept between calls (src/sgbm_gpu.cu:119-145 resizes lazily
 // too). Nothing is zeroed per call: every buffer a stage reads was fully written by the stage
 // before it.
+#include "host_copy.h"
 #include "lumen_sgbm.h"
 #include "sgbm_host.h"
 #include "sgbm_internal.h"
@@ -397,10 +398,8 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
         bgr_to_gray(left, left_stride, W, H, pin);
         bgr_to_gray(right, right_stride, W, H, pin + n);
     } else {
-        for (int y = 0; y < H; ++y) {
-            std::memcpy(pin + (size_t)y * W, left + (size_t)y * left_stride, W);
-            std::memcpy(pin + n + (size_t)y * W, right + (size_t)y * right_stride, W);
-        }
+        parallel_copy_rows(pin, W, left, left_stride, W, H);
+        parallel_copy_rows(pin + n, W, right, right_stride, W, H);
     }
     const int slot = begin_slot(h);
     e = hipEventRecord(h->ev[slot][0], s);
@@ -417,10 +416,7 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
     if (e != hipSuccess) return h->hip_fail(e, "download");
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return h->hip_fail(e, "compute");
-    const int16_t* pout = h->stageOut.as<int16_t>();
-    for (int y = 0; y < H; ++y)
-        std::memcpy(reinterpret_cast<uint8_t*>(out) + (size_t)y * out_stride, pout + (size_t)y * W,
-                    (size_t)W * 2);
+    parallel_copy_rows(out, out_stride, h->stageOut.ptr, (size_t)W * 2, (size_t)W * 2, H);
     h->haveResult = true;
     return LUMEN_OK;
 }

@@ -4,7 +4,8 @@
 // One dispatcher thread owns the device side. Each pass it
 //   1. admits waiting requests whose working set fits the pool (serve_core.h Admission),
 //   2. retires lanes whose batch has finished (outputs copied out, pool ranges released),
-//   3. starts every admitted request that has not started, as one batch, on each idle lane.
+//   3. starts admitted requests that have not started on each idle lane, as one batch of
+//      requests of similar size (pick_batch), so small ones never wait for a large one.
 // A lane is a stream with its own BatchEngine (persistent per-stage kernels in one hipGraph), so
 // batches on different lanes overlap on the device, and requests that arrive while a batch runs
 // start on the next free lane instead of waiting for it. Each pair's result is the batched
@@ -25,6 +26,7 @@
 #include <vector>
 
 #include "batch_engine.h"
+#include "host_copy.h"
 #include "lumen_sgbm.h"
 #include "serve_core.h"
 #include "sgbm_host.h"
@@ -55,6 +57,7 @@ struct Request {
     uint64_t id = 0;
     lumen_pair_desc d{};
     size_t bytes = 0;
+    double work = 0;  // algorithmic HBM bytes of the pair: W*H*Dp*(1+5*paths)
     char* region = nullptr;
     int16_t* devOut = nullptr;  // the region's output map (host-memory requests)
     size_t inOff = 0, outOff = 0;
@@ -159,10 +162,15 @@ int start_lane(lumen_server* S, Lane& L) {
         if (e == hipSuccess) e = grow_pinned(L.pinOut, L.pinOutCap, outBytes);
         if (e != hipSuccess) return fail(e, "pinned staging");
         uint8_t* pin = static_cast<uint8_t*>(L.pinIn);
+        std::vector<CopyJob> stage;
         for (Request* r : L.reqs) {
             const size_t n = (size_t)r->d.width * r->d.height;
-            std::memcpy(pin + r->inOff, r->d.left, n);
-            std::memcpy(pin + r->inOff + align_up(n, 256), r->d.right, n);
+            stage.push_back({pin + r->inOff, r->d.left, n});
+            stage.push_back({pin + r->inOff + align_up(n, 256), r->d.right, n});
+        }
+        parallel_copy(stage);
+        for (Request* r : L.reqs) {
+            const size_t n = (size_t)r->d.width * r->d.height;
             e = hipMemcpyAsync(r->region, pin + r->inOff, align_up(n, 256) + n,
                                hipMemcpyHostToDevice, s);
             if (e != hipSuccess) return fail(e, "upload");
@@ -194,8 +202,10 @@ void finish_lane(lumen_server* S, Lane& L, hipError_t status) {
     }
     if (S->cfg.device_io == 0) {
         const uint8_t* pout = static_cast<const uint8_t*>(L.pinOut);
+        std::vector<CopyJob> outs;
         for (Request* r : L.reqs)
-            std::memcpy(r->d.out, pout + r->outOff, (size_t)r->d.width * r->d.height * 2);
+            outs.push_back({r->d.out, pout + r->outOff, (size_t)r->d.width * r->d.height * 2});
+        parallel_copy(outs);
     }
 }
 
@@ -220,6 +230,8 @@ void retire_locked(lumen_server* S, Lane& L) {
 void dispatcher(lumen_server* S) {
     (void)hipSetDevice(S->device);
     const size_t maxBatch = static_cast<size_t>(S->cfg.max_batch);
+    const double budget = S->cfg.batch_budget_us * 5e6;  // bytes at 5 TB/s
+    std::vector<double> work;
     std::unique_lock<std::mutex> lk(S->m);
     for (;;) {
         bool progress = false;
@@ -244,10 +256,14 @@ void dispatcher(lumen_server* S) {
         }
         for (auto& L : S->lanes) {
             if (L->busy || S->ready.empty()) continue;
-            while (!S->ready.empty() && L->reqs.size() < maxBatch) {
-                L->reqs.push_back(S->ready.front());
-                S->ready.pop_front();
+            work.clear();
+            for (const Request* r : S->ready) work.push_back(r->work);
+            const std::vector<size_t> pick = pick_batch(work, maxBatch, budget, 4.0);
+            for (size_t k = pick.size(); k-- > 0;) {
+                L->reqs.push_back(S->ready[pick[k]]);
+                S->ready.erase(S->ready.begin() + static_cast<std::ptrdiff_t>(pick[k]));
             }
+            std::reverse(L->reqs.begin(), L->reqs.end());
             L->busy = true;
             ++S->busyLanes;
             ++S->stats.batches;
@@ -298,6 +314,7 @@ int lumen_server_create(const lumen_server_config* cfg, lumen_server** out) {
     if (c.lanes == 0) c.lanes = 2;
     if (c.max_batch == 0) c.max_batch = 64;
     if (c.max_bypass == 0) c.max_bypass = 8;
+    if (c.batch_budget_us <= 0) c.batch_budget_us = 2000;
     if (c.lanes < 1 || c.lanes > 4 || c.max_batch < 1 || c.max_batch > 65535) {
         g_serve_error = "lanes must be 1..4 and max_batch 1..65535";
         return LUMEN_EINVAL;
@@ -387,6 +404,9 @@ int lumen_server_submit(lumen_server* s, const lumen_pair_desc* pair, uint64_t* 
     }
     r->d = d;
     r->bytes = bytes;
+    r->work = static_cast<double>(d.width) * d.height *
+              padded_range(d.params.maxDisparity - d.params.minDisparity) *
+              (1 + 5 * num_paths(d.params));
     r->info.t_submit_ns = monotonic_ns();
     r->info.lane = -1;
     {

@@ -1,11 +1,14 @@
 // Host-only tests of the serving loop's pool and admission logic (serve_core.h), with an
-// injected backing allocator in place of hipMalloc. Exit status 0 = all passed.
+// injected backing allocator in place of hipMalloc, and of the staging copy pool
+// (host_copy.cpp). Exit status 0 = all passed.
 #include <cstdint>
+#include <algorithm>
 #include <cstdio>
 #include <random>
 #include <set>
 #include <vector>
 
+#include "host_copy.h"
 #include "serve_core.h"
 
 using namespace lumen;
@@ -221,6 +224,56 @@ static void percentiles() {
     EXPECT(percentile({3, 1, 2}, 50) == 2);
 }
 
+// The staging copy pool: many jobs of odd sizes, large enough to use the threads, twice in a
+// row (the pool is reused), and strided rows.
+static void parallel_copies() {
+    std::mt19937 rng(3);
+    std::vector<std::vector<uint8_t>> src, dst;
+    std::vector<CopyJob> jobs;
+    for (int i = 0; i < 40; ++i) {
+        const size_t n = 1 + rng() % (700 * 1024);
+        src.emplace_back(n);
+        for (auto& b : src.back()) b = static_cast<uint8_t>(rng());
+        dst.emplace_back(n, 0);
+    }
+    for (int rep = 0; rep < 2; ++rep) {
+        jobs.clear();
+        for (size_t i = 0; i < src.size(); ++i) {
+            std::fill(dst[i].begin(), dst[i].end(), 0);
+            jobs.push_back({dst[i].data(), src[i].data(), src[i].size()});
+        }
+        parallel_copy(jobs);
+        for (size_t i = 0; i < src.size(); ++i) EXPECT(dst[i] == src[i]);
+    }
+    const int W = 1000, H = 3000, sp = 1200, dp = 1100;
+    std::vector<uint8_t> a((size_t)sp * H), b((size_t)dp * H, 7);
+    for (auto& v : a) v = static_cast<uint8_t>(rng());
+    parallel_copy_rows(b.data(), dp, a.data(), sp, W, H);
+    bool ok = true;
+    for (int y = 0; y < H && ok; ++y)
+        for (int x = 0; x < dp; ++x) {
+            const uint8_t want = x < W ? a[(size_t)y * sp + x] : 7;
+            if (b[(size_t)y * dp + x] != want) ok = false;
+        }
+    EXPECT(ok);
+}
+
+static void batch_picking() {
+    // Oldest first; only requests within 4x of the first's work; total within the budget.
+    std::vector<double> w = {10, 100, 12, 3, 30, 41, 9, 11};
+    auto p = pick_batch(w, 64, 50, 4.0);
+    EXPECT((p == std::vector<size_t>{0, 2, 3, 6, 7}));  // 10+12+3+9+11 = 45; 30 would exceed 50
+    p = pick_batch(w, 2, 1000, 4.0);
+    EXPECT((p == std::vector<size_t>{0, 2}));         // count cap
+    w = {100, 10, 30, 400, 90};
+    p = pick_batch(w, 64, 50, 4.0);
+    EXPECT((p == std::vector<size_t>{0}));            // a large first request: 90 would exceed max(50, 100)
+    p = pick_batch(w, 64, 500, 4.0);
+    EXPECT((p == std::vector<size_t>{0, 2, 4}));      // 10 too small; 400 would exceed 500
+    EXPECT(pick_batch({}, 64, 50, 4.0).empty());
+    EXPECT((pick_batch({5}, 64, 1, 4.0) == std::vector<size_t>{0}));  // always at least one
+}
+
 int main() {
     struct {
         const char* name;
@@ -232,6 +285,8 @@ int main() {
         {"admission_fifo_and_bypass", admission_fifo_and_bypass},
         {"admission_simulation", admission_simulation},
         {"percentiles", percentiles},
+        {"parallel_copies", parallel_copies},
+        {"batch_picking", batch_picking},
     };
     for (auto& c : cases) {
         const int before = g_failed;
