@@ -147,6 +147,25 @@ __global__ __launch_bounds__(256) void batch_paths_kernel(const PairJob* __restr
 }
 
 // ----------------------------------------------------------------------------------- WTA
+template <int KSET, int NP>
+__device__ __forceinline__ void wta_dispatch(const PairJob& j, int y, unsigned long long* rbest) {
+    if constexpr (KSET == 0) {
+        switch (j.K) {
+            case 4: wta_row<4, true, NP>(j, y, rbest); break;
+            case 8: wta_row<8, true, NP>(j, y, rbest); break;
+            default: wta_row<12, true, NP>(j, y, rbest); break;
+        }
+    } else {
+        switch (j.K) {
+            case 16: wta_row<16, true, NP>(j, y, rbest); break;
+            case 20: wta_row<20, true, NP>(j, y, rbest); break;
+            case 24: wta_row<24, true, NP>(j, y, rbest); break;
+            case 28: wta_row<28, true, NP>(j, y, rbest); break;
+            default: wta_row<32, true, NP>(j, y, rbest); break;
+        }
+    }
+}
+
 // Units: rows of the pairs in this K set (prefix sums over those pairs only).
 template <int KSET>
 __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restrict__ jobs,
@@ -161,21 +180,7 @@ __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restric
         if (c.seek(start, npairs, u)) c.job = jobs[pairIdx[c.idx]];
         const PairJob& j = c.job;
         const int y = u - c.beg;
-        if constexpr (KSET == 0) {
-            switch (j.K) {
-                case 4: wta_row<4, true>(j, y, rbest); break;
-                case 8: wta_row<8, true>(j, y, rbest); break;
-                default: wta_row<12, true>(j, y, rbest); break;
-            }
-        } else {
-            switch (j.K) {
-                case 16: wta_row<16, true>(j, y, rbest); break;
-                case 20: wta_row<20, true>(j, y, rbest); break;
-                case 24: wta_row<24, true>(j, y, rbest); break;
-                case 28: wta_row<28, true>(j, y, rbest); break;
-                default: wta_row<32, true>(j, y, rbest); break;
-            }
-        }
+        wta_dispatch<KSET, 0>(j, y, rbest);
         __syncthreads();
     }
 }
@@ -196,7 +201,7 @@ __global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __res
         const int ty = local / tiles, tx = local - ty * tiles;
         const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
         if (x < j.W && y < j.H)
-            speckle_px(j.dispLR, j.spk, j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
+            speckle_px(gmem(j.dispLR), gmem(j.spk), j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
     }
 }
 
@@ -213,7 +218,7 @@ __global__ __launch_bounds__(256) void batch_median_kernel(const PairJob* __rest
         const int tiles = (j.W + kMedTW - 1) / kMedTW;
         const int local = u - c.beg;
         const int ty = local / tiles, tx = local - ty * tiles;
-        median_tile(j.doSpeckle ? j.spk : j.dispLR, j.out, j.W, j.H, tx * kMedTW, ty * kMedTH, tile);
+        median_tile(gmem(j.doSpeckle ? j.spk : j.dispLR), gmem(j.out), j.W, j.H, tx * kMedTW, ty * kMedTH, tile);
         __syncthreads();
     }
 }

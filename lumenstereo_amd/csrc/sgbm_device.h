@@ -34,11 +34,23 @@ __device__ __forceinline__ u16x2 as16(uint32_t v) { return __builtin_bit_cast(u1
 __device__ __forceinline__ u16x2 splat(uint32_t v) { return as16((v & 0xFFFFu) * 0x10001u); }
 __device__ __forceinline__ u16x2 vmin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
 
+// Global-memory view of a pointer. The stage bodies dereference the PairJob's buffers through
+// it: a PairJob read from device memory (the batched kernels' pair table) carries generic
+// pointers, for which the compiler emits flat accesses, and flat accesses also count against
+// the LDS wait counter, so every LDS wait of a stage would wait for them as well.
+#define LUMEN_G __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ LUMEN_G T* gmem(T* p) {
+    return (LUMEN_G T*)p;
+}
+
 // DPP controls (GFX9 encoding).
 constexpr int kQuadXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kQuadXor2 = 0x4E;        // quad_perm [2,3,0,1]
 constexpr int kRowShl1 = 0x101;        // lane i reads lane i+1 of its row
 constexpr int kRowShr1 = 0x111;        // lane i reads lane i-1 of its row
+constexpr int kRowRor1 = 0x121;        // lane i reads lane (i-1) mod 16 of its row
+constexpr int kRowRor15 = 0x12F;       // lane i reads lane (i+1) mod 16 of its row
 constexpr int kRowMirror = 0x140;      // lane i reads lane 15-i of its row
 constexpr int kRowHalfMirror = 0x141;  // lane i reads lane 7-i of its half row
 
@@ -166,8 +178,8 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
     if (rowOk) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) {
-            const uint8_t* lrow = j.left + (size_t)(y - 2 + r) * W;
-            const uint8_t* rrow = j.right + (size_t)(y - 2 + r) * W;
+            const LUMEN_G uint8_t* lrow = gmem(j.left) + (size_t)(y - 2 + r) * W;
+            const LUMEN_G uint8_t* rrow = gmem(j.right) + (size_t)(y - 2 + r) * W;
             for (int i = tid; i < lpitch; i += nt) {
                 const int c = x0 - 2 + i;
                 il[r * lpitch + i] = (c >= 0 && c < W) ? lrow[c] : uint8_t(0);
@@ -238,7 +250,7 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
                 w[q] = word;
             }
         }
-        *reinterpret_cast<u32x4*>(j.cost + ((size_t)y * W + x) * Dp + d0) =
+        *reinterpret_cast<LUMEN_G u32x4*>(gmem(j.cost) + ((size_t)y * W + x) * Dp + d0) =
             u32x4{w[0], w[1], w[2], w[3]};
     }
 }
@@ -249,7 +261,7 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
 // prefiltered rows y-r..y+r of the segment and of the right columns it can reach are built in
 // LDS. The valid taps of a window form a rectangle, so count = rows * columns.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ uint8_t prefilter_at(const uint8_t* __restrict__ row, int c, int W,
+__device__ __forceinline__ uint8_t prefilter_at(const LUMEN_G uint8_t* __restrict__ row, int c, int W,
                                                 int cap) {
     int g;
     if (c > 0 && c < W - 1)
@@ -288,13 +300,13 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
         const int c = i / (nq * 4), k = i - c * (nq * 4);
         const int gc = lbase + c, yy = y - r + k;
         lb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
-                    ? prefilter_at(j.left + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+                    ? prefilter_at(gmem(j.left) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
     }
     for (int i = tid; i < TR * nq * 4; i += nt) {
         const int c = i / (nq * 4), k = i - c * (nq * 4);
         const int gc = rbase + c, yy = y - r + k;
         rb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
-                    ? prefilter_at(j.right + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+                    ? prefilter_at(gmem(j.right) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
     }
     __syncthreads();
     const int dylo = max(-r, -y), dyhi = min(r, H - 1 - y);
@@ -379,7 +391,7 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     // The segment's costs are contiguous in the volume: 16-byte stores.
     const int npix = min(kSadTile, W - x0);
     const int n16 = npix * Dp / 16;
-    u32x4* dst = reinterpret_cast<u32x4*>(j.cost + ((size_t)y * W + x0) * Dp);
+    LUMEN_G u32x4* dst = reinterpret_cast<LUMEN_G u32x4*>(gmem(j.cost) + ((size_t)y * W + x0) * Dp);
     const u32x4* src = reinterpret_cast<const u32x4*>(ot);
     for (int i = tid; i < n16; i += nt) dst[i] = src[i];
 }
@@ -406,8 +418,8 @@ __device__ __forceinline__ uint8_t sad_cost_direct(const PairJob& j, int x, int 
     if (dyhi < dylo || dxhi < dxlo) return 255;
     int sum = 0;
     for (int dy = dylo; dy <= dyhi; ++dy) {
-        const uint8_t* lr = j.left + (size_t)(y + dy) * W;
-        const uint8_t* rr = j.right + (size_t)(y + dy) * W;
+        const LUMEN_G uint8_t* lr = gmem(j.left) + (size_t)(y + dy) * W;
+        const LUMEN_G uint8_t* rr = gmem(j.right) + (size_t)(y + dy) * W;
         for (int dx = dxlo; dx <= dxhi; ++dx)
             sum += abs(int(prefilter_at(lr, x + dx, W, cap)) - int(prefilter_at(rr, x + dx - a, W, cap)));
     }
@@ -476,57 +488,72 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
     step = dy * W + dx;
 }
 
+// Slice layout. Lane l of a DPP row holds K of a pixel's Dp = 16K disparities, in K/Q chunks of
+// Q consecutive ones (Q = 8 when K is a multiple of 8, else 4): chunk c is d = 16Qc + Ql ..
+// 16Qc + Ql + Q-1. A chunk is one 4- or 8-byte cost load and one 8- or 16-byte path-cost access
+// per lane, and the 16 lanes' chunks are contiguous, so every access instruction of a row
+// covers one contiguous 64 to 256 bytes whatever K is. For K <= 8 there is one chunk and lane l
+// simply holds d = lK .. lK+K-1. Registers hold the lane's elements chunk by chunk, element e
+// being disparity slice_d<K>(l, e).
+template <int K>
+__host__ __device__ constexpr int slice_q() { return K % 8 == 0 ? 8 : 4; }
+
+template <int K>
+__device__ __forceinline__ int slice_d(int l, int e) {
+    constexpr int Q = slice_q<K>();
+    return 16 * Q * (e / Q) + Q * l + e % Q;
+}
+
+// Lane that holds disparity d, and its element index there.
+template <int K>
+__device__ __forceinline__ int slice_lane(int d) {
+    constexpr int Q = slice_q<K>();
+    return (d % (16 * Q)) / Q;
+}
+template <int K>
+__device__ __forceinline__ int slice_elem(int d) {
+    constexpr int Q = slice_q<K>();
+    return d / (16 * Q) * Q + d % Q;
+}
+
 template <int K>
 struct CostBytes {
     uint32_t w[K / 4];
 };
 
+// p: the pixel's cost vector + Q*l (the lane's first chunk).
 template <int K>
-__device__ __forceinline__ CostBytes<K> load_cost(const uint8_t* p) {
+__device__ __forceinline__ CostBytes<K> load_cost(const LUMEN_G uint8_t* p) {
+    constexpr int Q = slice_q<K>();
     CostBytes<K> c;
-    if constexpr (K % 16 == 0) {
 #pragma unroll
-        for (int i = 0; i < K / 16; ++i) {
-            const u32x4 v = reinterpret_cast<const u32x4*>(p)[i];
-            c.w[4 * i + 0] = v.x;
-            c.w[4 * i + 1] = v.y;
-            c.w[4 * i + 2] = v.z;
-            c.w[4 * i + 3] = v.w;
+    for (int ch = 0; ch < K / Q; ++ch) {
+        if constexpr (Q == 8) {
+            const u32x2 v = *reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch);
+            c.w[2 * ch + 0] = v.x;
+            c.w[2 * ch + 1] = v.y;
+        } else {
+            c.w[ch] = *reinterpret_cast<const LUMEN_G uint32_t*>(p + 16 * Q * ch);
         }
-    } else if constexpr (K % 8 == 0) {
-#pragma unroll
-        for (int i = 0; i < K / 8; ++i) {
-            const u32x2 v = reinterpret_cast<const u32x2*>(p)[i];
-            c.w[2 * i + 0] = v.x;
-            c.w[2 * i + 1] = v.y;
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < K / 4; ++i) c.w[i] = reinterpret_cast<const uint32_t*>(p)[i];
     }
     return c;
 }
 
+// p: the pixel's path-cost vector + Q*l. Each chunk store writes a whole contiguous run of
+// the row's 16 lanes, so the stores are nontemporal.
 template <int K>
-__device__ __forceinline__ void store_path(uint16_t* p, const u16x2 (&v)[K / 2]) {
-    if constexpr (K % 8 == 0) {
+__device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)[K / 2]) {
+    constexpr int Q = slice_q<K>();
 #pragma unroll
-        for (int i = 0; i < K / 8; ++i)
-            __builtin_nontemporal_store(
-                u32x4{bits(v[4 * i]), bits(v[4 * i + 1]), bits(v[4 * i + 2]), bits(v[4 * i + 3])},
-                reinterpret_cast<u32x4*>(p) + i);
-    } else if constexpr (K >= 20) {
-        // 8-byte pieces at a stride of 2K >= 40 bytes: one store instruction covers a row's
-        // 32K bytes only sparsely, so the pieces are left in L2 to merge into whole lines
-        // (nontemporal pieces reach HBM as partial writes: 28 % more write traffic at K = 20).
-#pragma unroll
-        for (int i = 0; i < K / 4; ++i)
-            reinterpret_cast<u32x2*>(p)[i] = u32x2{bits(v[2 * i]), bits(v[2 * i + 1])};
-    } else {
-#pragma unroll
-        for (int i = 0; i < K / 4; ++i)
-            __builtin_nontemporal_store(u32x2{bits(v[2 * i]), bits(v[2 * i + 1])},
-                                        reinterpret_cast<u32x2*>(p) + i);
+    for (int ch = 0; ch < K / Q; ++ch) {
+        if constexpr (Q == 8) {
+            __builtin_nontemporal_store(u32x4{bits(v[4 * ch]), bits(v[4 * ch + 1]),
+                                              bits(v[4 * ch + 2]), bits(v[4 * ch + 3])},
+                                        reinterpret_cast<LUMEN_G u32x4*>(p + 16 * Q * ch));
+        } else {
+            __builtin_nontemporal_store(u32x2{bits(v[2 * ch]), bits(v[2 * ch + 1])},
+                                        reinterpret_cast<LUMEN_G u32x2*>(p + 16 * Q * ch));
+        }
     }
 }
 
@@ -540,19 +567,35 @@ __device__ __forceinline__ void expand_cost(const CostBytes<K>& c, u16x2 cs, u16
     }
 }
 
-// d-1 and d+1 neighbour vectors of a packed per-lane slice (lane l holds d = l*K .. l*K+K-1).
+// d-1 and d+1 neighbour vectors of a packed slice (lane l of its DPP row). Inside a chunk
+// they are the lane's own neighbouring elements. At a chunk's start, d-1 is the last element
+// of the same chunk of lane l-1, or for lane 0 the last element of the previous chunk of lane
+// 15; at a chunk's end, d+1 is the first element of the same chunk of lane l+1, or for lane 15
+// of the next chunk of lane 0: one row rotation per chunk each way, then a lane select. The
+// values lane 0 gets for d = -1 and lane 15 for d = Dp are masked by the caller.
 template <int K>
 __device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&dm1)[K / 2],
-                                           uint32_t (&dp1)[K / 2]) {
-    const uint32_t fromPrev = dpp<kRowShr1>(bits(v[K / 2 - 1]), 0xFFFFFFFFu);
-    const uint32_t fromNext = dpp<kRowShl1>(bits(v[0]), 0xFFFFFFFFu);
+                                           uint32_t (&dp1)[K / 2], int l) {
+    constexpr int Q = slice_q<K>(), NC = K / Q, PQ = Q / 2;
+    uint32_t prevLast[NC], nextFirst[NC];
 #pragma unroll
-    for (int j = 0; j < K / 2; ++j) {
-        const uint32_t cur = bits(v[j]);
-        const uint32_t lo = j == 0 ? fromPrev : bits(v[j - 1]);
-        const uint32_t hi = j == K / 2 - 1 ? fromNext : bits(v[j + 1]);
-        dm1[j] = (cur << 16) | (lo >> 16);
-        dp1[j] = (cur >> 16) | (hi << 16);
+    for (int c = 0; c < NC; ++c) {
+        prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]), 0u);
+        nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]), 0u);
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint32_t fromPrev = (c > 0 && l == 0) ? prevLast[c - 1] : prevLast[c];
+        const uint32_t fromNext = (c < NC - 1 && l == 15) ? nextFirst[c + 1] : nextFirst[c];
+#pragma unroll
+        for (int jj = 0; jj < PQ; ++jj) {
+            const int q = c * PQ + jj;
+            const uint32_t cur = bits(v[q]);
+            const uint32_t lo = jj == 0 ? fromPrev : bits(v[q - 1]);
+            const uint32_t hi = jj == PQ - 1 ? fromNext : bits(v[q + 1]);
+            dm1[q] = (cur << 16) | (lo >> 16);
+            dp1[q] = (cur >> 16) | (hi << 16);
+        }
     }
 }
 
@@ -608,25 +651,27 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     const P2Table p2t = make_p2(j.P1, j.P2, lane);
 
     const int Dp = j.Dp, D = j.D;
-    const uint8_t* cbase = j.cost + l * K;
-    uint16_t* Lbase = j.L + (size_t)path * j.pathStride + l * K;
+    constexpr int Q = slice_q<K>();
+    const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
+    LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
     const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
     const u16x2 P1p = splat(static_cast<uint32_t>(j.P1));
     const uint32_t m0 = l == 0 ? 0xFFFFu : 0u;  // slot d = 0 has no d-1 neighbour
     uint32_t c2m[K / 2], padm[K / 2];
 #pragma unroll
     for (int q = 0; q < K / 2; ++q) {
-        const int dlo = l * K + 2 * q;
+        const int dlo = slice_d<K>(l, 2 * q);
         c2m[q] = (dlo == D - 1 ? 0xFFFFu : 0u) | (dlo + 1 == D - 1 ? 0xFFFF0000u : 0u);
         padm[q] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
     }
 
     // Origin: L = t16(C * costScale).
     const long long cstep = (long long)step * Dp;  // bytes between consecutive cost vectors
-    const uint8_t* lp = cbase + (size_t)p0 * Dp;   // cost pointer of the last issued load
+    const LUMEN_G uint8_t* lp = cbase + (size_t)p0 * Dp;  // cost pointer of the last issued load
     int lt = 0;                                     // step index of the last issued load
-    uint16_t* sp = Lbase + (size_t)p0 * Dp;         // path-cost pointer of the current step
-    uint16_t* const trash = trash_base + lane * K;  // stores of finished rows land here
+    LUMEN_G uint16_t* sp = Lbase + (size_t)p0 * Dp;         // path-cost pointer of the step
+    // Stores of finished rows land in the trash: one Dp-sized vector per DPP row.
+    LUMEN_G uint16_t* const trash = gmem(trash_base) + g * 16 * K + l * Q;
     u16x2 Lv[K / 2];
     {
         const CostBytes<K> c0 = load_cost<K>(lp);
@@ -649,8 +694,8 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     const int lu = l & (PF - 1);
     auto fetch = [&](int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
         const int ti = min(tb + lu, last);
-        iv[0] = j.left[p0 + ti * step];
-        iv[1] = j.left[p0 + max(ti - 1, 0) * step];
+        iv[0] = gmem(j.left)[p0 + ti * step];
+        iv[1] = gmem(j.left)[p0 + max(ti - 1, 0) * step];
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const bool adv = lt < last;
@@ -671,7 +716,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
             u16x2 Ch[K / 2];
             expand_cost<K>(blk[u], cs, Ch);
             uint32_t dm1[K / 2], dp1[K / 2];
-            neighbours<K>(Lv, dm1, dp1);
+            neighbours<K>(Lv, dm1, dp1, l);
 #pragma unroll
             for (int q = 0; q < K / 2; ++q) {
                 uint32_t c1 = bits(as16(dm1[q]) + P1p);
@@ -741,7 +786,7 @@ __device__ __forceinline__ void set_priority_for_length(int len) {
 // ---------------------------------------------------------------------------------------
 template <int K>
 __host__ __device__ inline size_t wta_lds_bytes(int W) {
-    return (size_t)(W + W / K + 1) * 8 + (size_t)W * 4;
+    return (size_t)(W + W / slice_q<K>() + 1) * 8 + (size_t)W * 4;
 }
 
 template <int K>
@@ -755,14 +800,136 @@ __device__ __forceinline__ uint32_t pick32(const uint32_t (&v)[K / 2], int slot)
     return r;
 }
 
-template <int K, bool PAD>
+// One pixel of a row for the DPP row of lane l: sums the NR path-cost vectors in `raw`
+// (mod 2^16), finds the left winner and second minimum, writes the left maps, and offers every
+// (pixel, d) to its right pixel in `rbest`.
+template <int K, bool PAD, int NR>
+__device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t rowPix,
+                                          const uint32_t (&raw)[NR][K / 2],
+                                          const uint32_t (&padm)[K / 2], int16_t* sdl,
+                                          unsigned long long* rbest) {
+    constexpr int Q = slice_q<K>();
+    const int D = j.D, m = j.m;
+    u16x2 S[K / 2];
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+        S[q] = as16(raw[0][q]);
+#pragma unroll
+        for (int r = 1; r < NR; ++r) S[q] += as16(raw[r][q]);
+    }
+    if constexpr (PAD) {
+#pragma unroll
+        for (int q = 0; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
+    }
+    uint32_t sm1[K / 2], sp1[K / 2];
+    neighbours<K>(S, sm1, sp1, l);
+
+    // Left WTA: (first minimum, second minimum of the multiset).
+    uint32_t k1 = 0xFFFFFFFFu, m2 = 0xFFFFu;
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
+            const uint32_t key = (v << 16) | static_cast<uint32_t>(slice_d<K>(l, 2 * q + h));
+            m2 = min(m2, max(k1, key) >> 16);
+            k1 = min(k1, key);
+        }
+    }
+#define LUMEN_WTA_COMBINE(CTRL)                   \
+    {                                             \
+        const uint32_t ok = dpp<CTRL>(k1, k1);    \
+        const uint32_t om = dpp<CTRL>(m2, m2);    \
+        m2 = min(min(m2, om), max(k1, ok) >> 16); \
+        k1 = min(k1, ok);                         \
+    }
+    LUMEN_WTA_COMBINE(kQuadXor1)
+    LUMEN_WTA_COMBINE(kQuadXor2)
+    LUMEN_WTA_COMBINE(kRowHalfMirror)
+    LUMEN_WTA_COMBINE(kRowMirror)
+#undef LUMEN_WTA_COMBINE
+    const int bestD = static_cast<int>(k1 & 0xFFFFu);
+    const uint32_t best = k1 >> 16;
+    if (l == slice_lane<K>(bestD)) {
+        int16_t out;
+        const uint32_t ratio = static_cast<uint32_t>(j.uniquenessRatio);
+        if (j.uniquenessRatio > 0 && m2 != 0xFFFFu && m2 <= best + (best * ratio) / 100u) {
+            out = -1;
+        } else {
+            float sub = static_cast<float>(bestD);
+            if (bestD > 0 && bestD < D - 1) {
+                const int slot = slice_elem<K>(bestD);
+                sub = parabola(sub, pick32<K>(sm1, slot), best, pick32<K>(sp1, slot));
+            }
+            out = f2i16((sub + static_cast<float>(m)) * 16.0f);
+        }
+        gmem(j.dispL)[rowPix + x] = out;
+        gmem(j.wtaInt)[rowPix + x] = static_cast<int16_t>(bestD);
+        sdl[x] = out;
+    }
+
+    // Right WTA candidates.
+#pragma unroll
+    for (int q = 0; q < K / 2; ++q) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int d = slice_d<K>(l, 2 * q + h);
+            const int xr = x - m - d;
+            const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
+            if (d < D && xr >= 0 && v != 0xFFFFu) {
+                const uint32_t lo = h ? (sm1[q] >> 16) : (sm1[q] & 0xFFFFu);
+                const uint32_t hi = h ? (sp1[q] >> 16) : (sp1[q] & 0xFFFFu);
+                const unsigned long long key = (static_cast<unsigned long long>(v) << 48) |
+                                               (static_cast<unsigned long long>(d) << 32) |
+                                               (lo << 16) | hi;
+                atomicMin(&rbest[xr + xr / Q], key);
+            }
+        }
+    }
+}
+
+// The NP path-cost vectors of one pixel for lane l (chunk by chunk, see the slice layout).
+template <int K, int NP>
+__device__ __forceinline__ void wta_load(const PairJob& j, size_t pix, int l,
+                                         uint32_t (&raw)[NP][K / 2]) {
+    constexpr int Q = slice_q<K>();
+    const LUMEN_G uint16_t* src = gmem(j.L) + pix * j.Dp + l * Q;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+        const LUMEN_G uint16_t* p = src + (size_t)r * j.pathStride;
+#pragma unroll
+        for (int ch = 0; ch < K / Q; ++ch) {
+            if constexpr (Q == 8) {
+                const u32x4 v =
+                    __builtin_nontemporal_load(reinterpret_cast<const LUMEN_G u32x4*>(p + 16 * Q * ch));
+                raw[r][4 * ch + 0] = v.x;
+                raw[r][4 * ch + 1] = v.y;
+                raw[r][4 * ch + 2] = v.z;
+                raw[r][4 * ch + 3] = v.w;
+            } else {
+                const u32x2 v =
+                    __builtin_nontemporal_load(reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch));
+                raw[r][2 * ch + 0] = v.x;
+                raw[r][2 * ch + 1] = v.y;
+            }
+        }
+    }
+}
+
+// NP is the path count (4 or 8): a compile-time count puts every plane's load of a pixel in
+// flight before the first add, and while a pixel's path costs fit 32 registers the pixel loop
+// is software-pipelined (the next pixel's loads are issued before the current pixel is
+// processed). NP = 0 takes the count from the job and adds each plane as it arrives: fewer
+// registers, for the batched kernel, which holds every slice width in one kernel.
+template <int K, bool PAD, int NP>
 __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
-    // rbest[xr + xr/K]: running right-WTA key of right pixel xr. One padding entry every K
-    // entries spreads the 16 lanes of a row (whose candidates are K right pixels apart) over
+    // rbest[xr + xr/Q]: running right-WTA key of right pixel xr. One padding entry every Q
+    // entries spreads the 16 lanes of a row (whose candidates are Q right pixels apart) over
     // distinct LDS banks.
-    const int W = j.W, D = j.D, Dp = j.Dp, m = j.m;
+    constexpr int Q = slice_q<K>();
+    const int W = j.W, D = j.D, m = j.m;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nslots = W + W / K + 1;
+    const int nslots = W + W / Q + 1;
     int16_t* sdl = reinterpret_cast<int16_t*>(rbest + nslots);
     int16_t* sdr = sdl + W;
     for (int i = tid; i < nslots; i += nt) rbest[i] = ~0ull;
@@ -771,108 +938,59 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
     uint32_t padm[K / 2];
 #pragma unroll
     for (int q = 0; q < K / 2; ++q) {
-        const int dlo = l * K + 2 * q;
+        const int dlo = slice_d<K>(l, 2 * q);
         padm[q] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
     }
     const size_t rowPix = (size_t)y * W;
-    for (int x = g; x < W; x += ngroups) {
-        const uint16_t* src = j.L + (rowPix + x) * Dp + l * K;
-        u16x2 S[K / 2];
+    if constexpr (NP == 0) {
+        for (int x = g; x < W; x += ngroups) {
+            const LUMEN_G uint16_t* src = gmem(j.L) + (rowPix + x) * j.Dp + l * Q;
+            uint32_t acc[1][K / 2];
 #pragma unroll
-        for (int q = 0; q < K / 2; ++q) S[q] = as16(0u);
-        for (int r = 0; r < j.npaths; ++r) {
-            const uint16_t* p = src + (size_t)r * j.pathStride;
-            if constexpr (K % 8 == 0) {
+            for (int q = 0; q < K / 2; ++q) acc[0][q] = 0u;
+            for (int r = 0; r < j.npaths; ++r) {
+                const LUMEN_G uint16_t* p = src + (size_t)r * j.pathStride;
 #pragma unroll
-                for (int i = 0; i < K / 8; ++i) {
-                    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p) + i);
-                    S[4 * i + 0] += as16(v.x);
-                    S[4 * i + 1] += as16(v.y);
-                    S[4 * i + 2] += as16(v.z);
-                    S[4 * i + 3] += as16(v.w);
-                }
-            } else {
-#pragma unroll
-                for (int i = 0; i < K / 4; ++i) {
-                    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p) + i);
-                    S[2 * i + 0] += as16(v.x);
-                    S[2 * i + 1] += as16(v.y);
-                }
-            }
-        }
-        if constexpr (PAD) {
-#pragma unroll
-            for (int q = 0; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
-        }
-        uint32_t sm1[K / 2], sp1[K / 2];
-        neighbours<K>(S, sm1, sp1);
-
-        // Left WTA: (first minimum, second minimum of the multiset).
-        uint32_t k1 = 0xFFFFFFFFu, m2 = 0xFFFFu;
-#pragma unroll
-        for (int q = 0; q < K / 2; ++q) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
-                const uint32_t key = (v << 16) | static_cast<uint32_t>(l * K + 2 * q + h);
-                m2 = min(m2, max(k1, key) >> 16);
-                k1 = min(k1, key);
-            }
-        }
-#define LUMEN_WTA_COMBINE(CTRL)                   \
-    {                                             \
-        const uint32_t ok = dpp<CTRL>(k1, k1);    \
-        const uint32_t om = dpp<CTRL>(m2, m2);    \
-        m2 = min(min(m2, om), max(k1, ok) >> 16); \
-        k1 = min(k1, ok);                         \
-    }
-        LUMEN_WTA_COMBINE(kQuadXor1)
-        LUMEN_WTA_COMBINE(kQuadXor2)
-        LUMEN_WTA_COMBINE(kRowHalfMirror)
-        LUMEN_WTA_COMBINE(kRowMirror)
-#undef LUMEN_WTA_COMBINE
-        const int bestD = static_cast<int>(k1 & 0xFFFFu);
-        const uint32_t best = k1 >> 16;
-        if (l == bestD / K) {
-            int16_t out;
-            const uint32_t ratio = static_cast<uint32_t>(j.uniquenessRatio);
-            if (j.uniquenessRatio > 0 && m2 != 0xFFFFu && m2 <= best + (best * ratio) / 100u) {
-                out = -1;
-            } else {
-                float sub = static_cast<float>(bestD);
-                if (bestD > 0 && bestD < D - 1) {
-                    const int slot = bestD - l * K;
-                    sub = parabola(sub, pick32<K>(sm1, slot), best, pick32<K>(sp1, slot));
-                }
-                out = f2i16((sub + static_cast<float>(m)) * 16.0f);
-            }
-            j.dispL[rowPix + x] = out;
-            j.wtaInt[rowPix + x] = static_cast<int16_t>(bestD);
-            sdl[x] = out;
-        }
-
-        // Right WTA candidates.
-#pragma unroll
-        for (int q = 0; q < K / 2; ++q) {
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int d = l * K + 2 * q + h;
-                const int xr = x - m - d;
-                const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
-                if (d < D && xr >= 0 && v != 0xFFFFu) {
-                    const uint32_t lo = h ? (sm1[q] >> 16) : (sm1[q] & 0xFFFFu);
-                    const uint32_t hi = h ? (sp1[q] >> 16) : (sp1[q] & 0xFFFFu);
-                    const unsigned long long key = (static_cast<unsigned long long>(v) << 48) |
-                                                   (static_cast<unsigned long long>(d) << 32) |
-                                                   (lo << 16) | hi;
-                    atomicMin(&rbest[xr + xr / K], key);
+                for (int ch = 0; ch < K / Q; ++ch) {
+                    if constexpr (Q == 8) {
+                        const u32x4 v = __builtin_nontemporal_load(
+                            reinterpret_cast<const LUMEN_G u32x4*>(p + 16 * Q * ch));
+                        acc[0][4 * ch + 0] = bits(as16(acc[0][4 * ch + 0]) + as16(v.x));
+                        acc[0][4 * ch + 1] = bits(as16(acc[0][4 * ch + 1]) + as16(v.y));
+                        acc[0][4 * ch + 2] = bits(as16(acc[0][4 * ch + 2]) + as16(v.z));
+                        acc[0][4 * ch + 3] = bits(as16(acc[0][4 * ch + 3]) + as16(v.w));
+                    } else {
+                        const u32x2 v = __builtin_nontemporal_load(
+                            reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch));
+                        acc[0][2 * ch + 0] = bits(as16(acc[0][2 * ch + 0]) + as16(v.x));
+                        acc[0][2 * ch + 1] = bits(as16(acc[0][2 * ch + 1]) + as16(v.y));
+                    }
                 }
             }
+            wta_pixel<K, PAD, 1>(j, x, l, rowPix, acc, padm, sdl, rbest);
+        }
+    } else if constexpr (NP * K / 2 <= 32) {
+        uint32_t bufA[NP][K / 2], bufB[NP][K / 2];
+        int x = g;
+        if (x < W) wta_load<K, NP>(j, rowPix + x, l, bufA);
+        for (; x < W; x += 2 * ngroups) {
+            const int x1 = x + ngroups, x2 = x1 + ngroups;
+            if (x1 < W) wta_load<K, NP>(j, rowPix + x1, l, bufB);
+            wta_pixel<K, PAD, NP>(j, x, l, rowPix, bufA, padm, sdl, rbest);
+            if (x2 < W) wta_load<K, NP>(j, rowPix + x2, l, bufA);
+            if (x1 < W) wta_pixel<K, PAD, NP>(j, x1, l, rowPix, bufB, padm, sdl, rbest);
+        }
+    } else {
+        // Wide slices: a second buffer would cost more occupancy than the overlap gains.
+        uint32_t buf[NP][K / 2];
+        for (int x = g; x < W; x += ngroups) {
+            wta_load<K, NP>(j, rowPix + x, l, buf);
+            wta_pixel<K, PAD, NP>(j, x, l, rowPix, buf, padm, sdl, rbest);
         }
     }
     __syncthreads();
     for (int xr = tid; xr < W; xr += nt) {
-        const unsigned long long key = rbest[xr + xr / K];
+        const unsigned long long key = rbest[xr + xr / Q];
         float sub = 0.0f;
         if (key != ~0ull) {
             const int bd = static_cast<int>((key >> 32) & 0xFFFFu);
@@ -882,7 +1000,7 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
                                static_cast<uint32_t>(key >> 48), static_cast<uint32_t>(key & 0xFFFFu));
         }
         const int16_t dr = f2i16((sub + static_cast<float>(m)) * 16.0f);
-        j.dispR[rowPix + xr] = dr;
+        gmem(j.dispR)[rowPix + xr] = dr;
         sdr[xr] = dr;
     }
     __syncthreads();
@@ -897,7 +1015,7 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
                 if (dr <= 0 || abs(int(dl) - int(dr)) > j.lrMaxDiff) dl = -1;
             }
         }
-        j.dispLR[rowPix + x] = dl;
+        gmem(j.dispLR)[rowPix + x] = dl;
     }
 }
 
@@ -905,7 +1023,8 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
 // Speckle filter (src/sgbm_kernels.cu:979-1013) with snapshot semantics: reads `in`, writes
 // `out`, so the result does not depend on thread timing. One pixel.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void speckle_px(const int16_t* __restrict__ in, int16_t* __restrict__ out,
+__device__ __forceinline__ void speckle_px(const LUMEN_G int16_t* __restrict__ in,
+                                           LUMEN_G int16_t* __restrict__ out,
                                            int W, int H, int x, int y, int minSimilar,
                                            int maxDiff) {
     const int16_t c = in[(size_t)y * W + x];
@@ -939,7 +1058,7 @@ __device__ __forceinline__ void cswap(int& a, int& b) {
     a = lo;
 }
 
-__device__ void median_tile(const int16_t* __restrict__ in, int16_t* __restrict__ out, int W,
+__device__ void median_tile(const LUMEN_G int16_t* __restrict__ in, LUMEN_G int16_t* __restrict__ out, int W,
                             int H, int x0, int y0, int16_t (*tile)[kMedTW + 2]) {
     for (int i = threadIdx.x; i < (kMedTH + 2) * (kMedTW + 2); i += blockDim.x) {
         const int ty = i / (kMedTW + 2), tx = i % (kMedTW + 2);

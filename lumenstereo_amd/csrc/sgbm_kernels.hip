@@ -34,7 +34,7 @@ __global__ __launch_bounds__(256) void cost_sad_direct_kernel(PairJob j) {
     if (item >= total) return;
     const int d = static_cast<int>(item % j.Dp);
     const size_t pix = item / j.Dp;
-    j.cost[item] = sad_cost_direct(j, static_cast<int>(pix % j.W), static_cast<int>(pix / j.W), d);
+    gmem(j.cost)[item] = sad_cost_direct(j, static_cast<int>(pix % j.W), static_cast<int>(pix / j.W), d);
 }
 
 hipError_t launch_cost(const PairJob& j, hipStream_t s) {
@@ -63,20 +63,25 @@ hipError_t launch_cost(const PairJob& j, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------- WTA
-template <int K, bool PAD>
+template <int K, bool PAD, int NP>
 __global__ __launch_bounds__(256) void wta_row_kernel(PairJob j) {
     extern __shared__ unsigned long long rbest[];
-    wta_row<K, PAD>(j, blockIdx.x, rbest);
+    wta_row<K, PAD, NP>(j, blockIdx.x, rbest);
+}
+
+template <int K, int NP>
+static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s) {
+    const size_t lds = wta_lds_bytes<K>(j.W);
+    if (j.D == j.Dp)
+        hipLaunchKernelGGL((wta_row_kernel<K, false, NP>), dim3(j.H), dim3(256), lds, s, j);
+    else
+        hipLaunchKernelGGL((wta_row_kernel<K, true, NP>), dim3(j.H), dim3(256), lds, s, j);
+    return hipGetLastError();
 }
 
 template <int K>
 static hipError_t launch_wta_k(const PairJob& j, hipStream_t s) {
-    const size_t lds = wta_lds_bytes<K>(j.W);
-    if (j.D == j.Dp)
-        hipLaunchKernelGGL((wta_row_kernel<K, false>), dim3(j.H), dim3(256), lds, s, j);
-    else
-        hipLaunchKernelGGL((wta_row_kernel<K, true>), dim3(j.H), dim3(256), lds, s, j);
-    return hipGetLastError();
+    return j.npaths == 8 ? launch_wta_kn<K, 8>(j, s) : launch_wta_kn<K, 4>(j, s);
 }
 
 hipError_t launch_wta(const PairJob& j, hipStream_t s) {
@@ -98,12 +103,12 @@ hipError_t launch_wta(const PairJob& j, hipStream_t s) {
 __global__ __launch_bounds__(256) void speckle_kernel(PairJob j) {
     const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (x >= j.W || y >= j.H) return;
-    speckle_px(j.dispLR, j.spk, j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
+    speckle_px(gmem(j.dispLR), gmem(j.spk), j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
 }
 
 __global__ __launch_bounds__(256) void median3x3_kernel(PairJob j) {
     __shared__ int16_t tile[kMedTH + 2][kMedTW + 2];
-    median_tile(j.doSpeckle ? j.spk : j.dispLR, j.out, j.W, j.H, blockIdx.x * kMedTW,
+    median_tile(gmem(j.doSpeckle ? j.spk : j.dispLR), gmem(j.out), j.W, j.H, blockIdx.x * kMedTW,
                 blockIdx.y * kMedTH, tile);
 }
 

@@ -29,6 +29,8 @@ CASES = [
     (70, 50, 0, 16, 0, 4, {"blockSize": 201}),   # SAD window too large for the LDS segment
     (150, 64, 0, 200, 1, 4, {}),          # 16 disparities per lane
     (120, 40, 5, 295, 0, 4, {}),          # D = 290: 20 per lane, padded to 320
+    (100, 30, 0, 420, 1, 4, {}),          # D = 420: 28 per lane (7 chunks of 4), padded to 448
+    (96, 24, 0, 512, 0, 8, {}),           # D = 512, the largest range: 32 per lane
 ]
 
 
