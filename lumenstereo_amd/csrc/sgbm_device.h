@@ -129,11 +129,16 @@ __device__ __forceinline__ float parabola(float s, uint32_t a0, uint32_t a1, uin
 constexpr int kCensusTile = 256;  // default pixels per census cost segment (batched form)
 constexpr int kSadTile = 64;      // pixels per SAD cost segment
 
-// LDS of a census segment of T pixels: left codes, the codes of every right pixel the segment
-// can reach, and the five staged rows of both images.
+// Census output is computed 64 pixels (one per lane) at a time into an LDS tile whose rows are
+// Dp + 16 bytes apart, so the 16-byte writes of 16 consecutive lanes start on 16 different
+// 4-bank groups.
+constexpr int kCensusPass = 64;
+
+// LDS of a census segment of T pixels: the output tile of one pass, left codes, the codes of
+// every right pixel the segment can reach, and the five staged rows of both images.
 __host__ __device__ inline size_t census_lds_bytes(int Dp, int T = kCensusTile) {
     const int nr = T + Dp - 1;
-    return (size_t)(T + nr) * 4 + 5 * (T + 4) + 5 * (nr + 4);
+    return (size_t)kCensusPass * (Dp + 16) + (size_t)(T + nr) * 4 + 5 * (T + 4) + 5 * (nr + 4);
 }
 
 // SAD segment LDS: prefiltered left/right tiles stored column-major (the 2r+1 rows of a column
@@ -163,17 +168,23 @@ __device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, in
 
 // Block-wide; every thread of the block must call it with the same (y, x0). T is the segment
 // width (a schedule choice; the costs do not depend on it).
+//
+// After the codes are in LDS the segment is produced in passes of 64 pixels: lane i of a wave
+// takes pixel p0 + i and the wave's 16-disparity chunk c, so the right codes the 64 lanes read
+// at each step are 64 consecutive words (no bank conflicts); the results go to the pass's LDS
+// tile, which the block then stores as one contiguous run of 16-byte pieces.
 template <int T = kCensusTile>
 __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* shc) {
     const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int rs = x0 - m - (Dp - 1);  // first right pixel the segment can reach
     const int nr = T + Dp - 1;         // right pixels the segment can reach
-    const int lpitch = T + 4, rpitch = nr + 4;
-    uint32_t* sl = shc;                                         // T codes
-    uint32_t* sr = shc + T;                                     // nr codes
-    uint8_t* il = reinterpret_cast<uint8_t*>(shc + T + nr);  // 5 x lpitch bytes
-    uint8_t* ir = il + 5 * lpitch;                              // 5 x rpitch bytes
+    const int lpitch = T + 4, rpitch = nr + 4, opitch = Dp + 16;
+    uint8_t* tile = reinterpret_cast<uint8_t*>(shc);                       // 64 x opitch bytes
+    uint32_t* sl = reinterpret_cast<uint32_t*>(tile + kCensusPass * opitch);  // T codes
+    uint32_t* sr = sl + T;                                                  // nr codes
+    uint8_t* il = reinterpret_cast<uint8_t*>(sr + nr);                      // 5 x lpitch bytes
+    uint8_t* ir = il + 5 * lpitch;                                          // 5 x rpitch bytes
     const bool rowOk = y >= 2 && y < H - 2;
     if (rowOk) {
 #pragma unroll
@@ -201,57 +212,60 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
         __syncthreads();
     }
     const int chunks = Dp / 16;
-    // item = pixel * chunks + chunk, advanced by nt items without a division per item.
-    int i = tid / chunks, c = tid - i * chunks;
-    const int di = nt / chunks, dc = nt - di * chunks;
-    for (; i < T; i += di, c += dc) {
-        if (c >= chunks) {
-            c -= chunks;
-            ++i;
-            if (i >= T) break;
-        }
+    const int lane = tid & 63, wave = tid >> 6, nwaves = nt >> 6;
+    for (int p0 = 0; p0 < T && x0 + p0 < W; p0 += kCensusPass) {
+        const int i = p0 + lane;
         const int x = x0 + i;
-        if (x >= W) break;
         const bool leftOk = rowOk && x >= 2 && x < W - 2;
         const uint32_t a = leftOk ? sl[i] : 0u;
-        const int d0 = c * 16;
-        const uint32_t* rc = sr + i + Dp - 1 - d0;  // rc[-k] is the right code for d0 + k
-        uint32_t w[4];
-        if (leftOk && d0 + 15 < D && x - m - d0 - 15 >= 2 && x - m - d0 < W - 2) {
-            // Interior: all 16 disparities valid. (h*255+12)/24 == (h*2718+168) >> 8 for
-            // h in [0,24]; two results per packed 16-bit multiply-add, byte 1 of each half
-            // gathered with one byte permute per four results.
+        for (int c = wave; c < chunks; c += nwaves) {
+            const int d0 = c * 16;
+            const uint32_t* rc = sr + i + Dp - 1 - d0;  // rc[-k] is the right code for d0 + k
+            uint32_t w[4];
+            if (leftOk && d0 + 15 < D && x - m - d0 - 15 >= 2 && x - m - d0 < W - 2) {
+                // Interior: all 16 disparities valid. (h*255+12)/24 == (h*2718+168) >> 8 for
+                // h in [0,24]; two results per packed 16-bit multiply-add, byte 1 of each half
+                // gathered with one byte permute per four results.
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const uint32_t h0 = __builtin_popcount(a ^ rc[-(4 * q + 0)]);
-                const uint32_t h1 = __builtin_popcount(a ^ rc[-(4 * q + 1)]);
-                const uint32_t h2 = __builtin_popcount(a ^ rc[-(4 * q + 2)]);
-                const uint32_t h3 = __builtin_popcount(a ^ rc[-(4 * q + 3)]);
-                const u16x2 k = splat(2718u), b = splat(168u);
-                const u16x2 v01 = as16(h0 | (h1 << 16)) * k + b;
-                const u16x2 v23 = as16(h2 | (h3 << 16)) * k + b;
-                w[q] = __builtin_amdgcn_perm(bits(v23), bits(v01), 0x07050301u);
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                uint32_t word = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int d = d0 + q * 4 + b;
-                    const int rx = x - m - d;
-                    uint32_t v = 255;
-                    if (leftOk && d < D && rx >= 2 && rx < W - 2) {
-                        const uint32_t h = __builtin_popcount(a ^ rc[-(4 * q + b)]);
-                        v = (h * 255u + 12u) / 24u;
-                    }
-                    word |= v << (8 * b);
+                for (int q = 0; q < 4; ++q) {
+                    const uint32_t h0 = __builtin_popcount(a ^ rc[-(4 * q + 0)]);
+                    const uint32_t h1 = __builtin_popcount(a ^ rc[-(4 * q + 1)]);
+                    const uint32_t h2 = __builtin_popcount(a ^ rc[-(4 * q + 2)]);
+                    const uint32_t h3 = __builtin_popcount(a ^ rc[-(4 * q + 3)]);
+                    const u16x2 k = splat(2718u), b = splat(168u);
+                    const u16x2 v01 = as16(h0 | (h1 << 16)) * k + b;
+                    const u16x2 v23 = as16(h2 | (h3 << 16)) * k + b;
+                    w[q] = __builtin_amdgcn_perm(bits(v23), bits(v01), 0x07050301u);
                 }
-                w[q] = word;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    uint32_t word = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int d = d0 + q * 4 + b;
+                        const int rx = x - m - d;
+                        uint32_t v = 255;
+                        if (leftOk && d < D && rx >= 2 && rx < W - 2) {
+                            const uint32_t h = __builtin_popcount(a ^ rc[-(4 * q + b)]);
+                            v = (h * 255u + 12u) / 24u;
+                        }
+                        word |= v << (8 * b);
+                    }
+                    w[q] = word;
+                }
             }
+            *reinterpret_cast<u32x4*>(tile + lane * opitch + d0) = u32x4{w[0], w[1], w[2], w[3]};
         }
-        *reinterpret_cast<LUMEN_G u32x4*>(gmem(j.cost) + ((size_t)y * W + x) * Dp + d0) =
-            u32x4{w[0], w[1], w[2], w[3]};
+        __syncthreads();
+        const int npx = min(kCensusPass, W - (x0 + p0));
+        LUMEN_G uint8_t* dst = gmem(j.cost) + ((size_t)y * W + x0 + p0) * Dp;
+        for (int f = tid; f < npx * chunks; f += nt) {
+            const int pi = f / chunks, c = f - pi * chunks;
+            *reinterpret_cast<LUMEN_G u32x4*>(dst + (size_t)f * 16) =
+                *reinterpret_cast<const u32x4*>(tile + pi * opitch + c * 16);
+        }
+        __syncthreads();
     }
 }
 
