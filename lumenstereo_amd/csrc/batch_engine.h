@@ -17,6 +17,22 @@
 
 namespace lumen {
 
+// Device-resident prefix tables of a batch, each with room for `cap` pairs: costStart,
+// wta0Start, wta1Start, spkStart, medStart ([cap+1] each), then wta0Pairs, wta1Pairs, spkPairs
+// ([cap] each); hdr holds the batch's sizes. Unit u of a stage belongs to the pair p with
+// start[p] <= u < start[p+1].
+struct Tables {
+    const int* hdr;
+    const int* costStart;
+    const int* wta0Start;
+    const int* wta1Start;
+    const int* spkStart;
+    const int* medStart;
+    const int* wta0Pairs;
+    const int* wta1Pairs;
+    const int* spkPairs;
+};
+
 // One aggregation unit: four consecutive scanlines of one path of one pair.
 struct PathItem {
     uint16_t pair;
@@ -77,6 +93,10 @@ private:
     size_t hostTablesCap_ = 0;
     hipGraphExec_t exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
+    // Nodes whose grid is set to the batch's unit count before each launch.
+    hipGraphNode_t costNode_ = nullptr;
+    hipGraphNode_t wtaNode_[2] = {nullptr, nullptr};
+    Tables argTables_{};                 // the tables argument of the graph's kernels
     size_t capCostLds_ = 0, capWta0Lds_ = 0, capWta1Lds_ = 0;
     const void* capPtrs_[4] = {nullptr, nullptr, nullptr, nullptr};
     bool capBig_ = false, capSpeckle_ = false;

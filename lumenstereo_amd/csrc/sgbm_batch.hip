@@ -2,11 +2,13 @@
 // mode and path count per stage, in persistent kernels driven by work lists, replayed as one
 // hipGraph.
 //
-// Each stage is one persistent kernel over a list of units of work. Aggregation waves take
-// their units from an atomic counter (the units differ in length by orders of magnitude and
-// are sorted longest first); the other stages' units are of similar size, so block b takes
-// units b, b + grid, b + 2 grid, ... with no atomics (hundreds of thousands of small units
-// would serialise on one counter):
+// Each stage is one kernel over a list of units of work. Aggregation waves are persistent and
+// take their units from an atomic counter (the units differ in length by orders of magnitude
+// and are sorted longest first). Cost and WTA run one block per unit: their grids are set to
+// the batch's unit counts on the instantiated graph before each launch, so the hardware
+// overlaps one block's row staging with other blocks' work as in the per-pair kernels. The
+// small post-processing units are strided over a fixed grid (block b takes units b,
+// b + grid, ...):
 //   cost        (pair, row, segment)       block-wide  census or SAD by the pair's mode
 //   paths       (pair, path, 4 scanlines)  wave-wide   list sorted by scanline length, longest
 //                                                      first, so the stage's tail stays short
@@ -39,19 +41,6 @@ enum Counter { kCtrPaths = 0, kCtrPathsBig, kNumCounters };
 // Device-resident sizes of the current batch (so one built graph serves any batch).
 enum HeaderField { kHdrPairs = 0, kHdrWta0, kHdrWta1, kHdrSpeckle, kHdrItems, kHdrItemsBig, kHdrFields };
 
-// Prefix tables, each with room for `cap` pairs: costStart, wta0Start, wta1Start, spkStart,
-// medStart ([cap+1] each), then wta0Pairs, wta1Pairs, spkPairs ([cap] each).
-struct Tables {
-    const int* hdr;
-    const int* costStart;
-    const int* wta0Start;
-    const int* wta1Start;
-    const int* spkStart;
-    const int* medStart;
-    const int* wta0Pairs;
-    const int* wta1Pairs;
-    const int* spkPairs;
-};
 
 // Prefix sums of per-pair unit counts: unit u belongs to the pair p with start[p] <= u < start[p+1].
 __device__ __forceinline__ int find_pair(const int* __restrict__ start, int npairs, int u) {
@@ -87,27 +76,27 @@ __global__ void reset_counters_kernel(int* ctr) {
 
 // ----------------------------------------------------------------------------------- cost
 // Units: census pairs contribute ceil(W/256)*H segments, SAD pairs ceil(W/64)*H.
+// One block per unit: the grid is set to the batch's unit count before each launch (the
+// graph's cost node is updated in place), so blocks are scheduled by the hardware like the
+// per-pair cost kernel's, with each block's row staging overlapping other blocks' work.
 __global__ __launch_bounds__(256) void batch_cost_kernel(const PairJob* __restrict__ jobs,
                                                          Tables t) {
     extern __shared__ uint32_t lds[];
     const int npairs = t.hdr[kHdrPairs];
     const int* start = t.costStart;
-    const int total = start[npairs];
-    PairCursor c;
-    for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        if (c.seek(start, npairs, u)) c.job = jobs[c.idx];
-        const PairJob& j = c.job;
-        const int local = u - c.beg;
-        if (j.mode == 1) {
-            const int segs = (j.W + kCensusTile - 1) / kCensusTile;
-            const int y = local / segs;
-            census_cost_segment(j, y, (local - y * segs) * kCensusTile, lds);
-        } else {
-            const int segs = (j.W + kSadTile - 1) / kSadTile;
-            const int y = local / segs;
-            sad_cost_segment(j, y, (local - y * segs) * kSadTile, reinterpret_cast<uint8_t*>(lds));
-        }
-        __syncthreads();
+    const int u = blockIdx.x;
+    if (u >= start[npairs]) return;
+    const int p = find_pair(start, npairs, u);
+    const PairJob j = jobs[p];
+    const int local = u - start[p];
+    if (j.mode == 1) {
+        const int segs = (j.W + kCensusTile - 1) / kCensusTile;
+        const int y = local / segs;
+        census_cost_segment(j, y, (local - y * segs) * kCensusTile, lds);
+    } else {
+        const int segs = (j.W + kSadTile - 1) / kSadTile;
+        const int y = local / segs;
+        sad_cost_segment(j, y, (local - y * segs) * kSadTile, reinterpret_cast<uint8_t*>(lds));
     }
 }
 
@@ -166,7 +155,8 @@ __device__ __forceinline__ void wta_dispatch(const PairJob& j, int y, unsigned l
     }
 }
 
-// Units: rows of the pairs in this K set (prefix sums over those pairs only).
+// Units: rows of the pairs in this K set (prefix sums over those pairs only), one block per
+// row like the per-pair kernel.
 template <int KSET>
 __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restrict__ jobs,
                                                         Tables t) {
@@ -174,15 +164,11 @@ __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restric
     const int npairs = t.hdr[KSET == 0 ? kHdrWta0 : kHdrWta1];
     const int* start = KSET == 0 ? t.wta0Start : t.wta1Start;
     const int* pairIdx = KSET == 0 ? t.wta0Pairs : t.wta1Pairs;
-    const int total = start[npairs];
-    PairCursor c;
-    for (int u = blockIdx.x; u < total; u += gridDim.x) {
-        if (c.seek(start, npairs, u)) c.job = jobs[pairIdx[c.idx]];
-        const PairJob& j = c.job;
-        const int y = u - c.beg;
-        wta_dispatch<KSET, 0>(j, y, rbest);
-        __syncthreads();
-    }
+    const int u = blockIdx.x;  // one row per block; the grid is set per launch
+    if (u >= start[npairs]) return;
+    const int q = find_pair(start, npairs, u);
+    const PairJob j = jobs[pairIdx[q]];
+    wta_dispatch<KSET, 0>(j, u - start[q], rbest);
 }
 
 // ----------------------------------------------------------------------------------- post
@@ -418,11 +404,8 @@ int BatchEngine::ensure_graph(const BatchPlan& pl, std::string& err) {
     const size_t wta1Lds = std::max<size_t>(std::max(pl.wta1Lds, capWta1Lds_), 16);
     const bool withBig = big || capBig_;
     const bool withSpk = spk || capSpeckle_;
-    const int gCost = occupancy_blocks(batch_cost_kernel, costLds);
     const int gPaths = occupancy_blocks(batch_paths_kernel<0>, 0);
     const int gPathsBig = occupancy_blocks(batch_paths_kernel<1>, 0);
-    const int gWta0 = occupancy_blocks(batch_wta_kernel<0>, wta0Lds);
-    const int gWta1 = occupancy_blocks(batch_wta_kernel<1>, wta1Lds);
     const int gSpk = occupancy_blocks(batch_speckle_kernel, 0);
     const int gMed = occupancy_blocks(batch_median_kernel, 0);
     // The graph is built node by node rather than by stream capture: a capture in progress
@@ -432,7 +415,7 @@ int BatchEngine::ensure_graph(const BatchPlan& pl, std::string& err) {
     if (e != hipSuccess) return fail(e, "graph create");
     hipGraphNode_t prev = nullptr;
     auto add = [&](const void* fn, int grid, int block, size_t lds, void** args) {
-        if (e != hipSuccess) return;
+        if (e != hipSuccess) return prev;
         hipKernelNodeParams kp{};
         kp.func = const_cast<void*>(fn);
         kp.gridDim = dim3(grid);
@@ -443,18 +426,22 @@ int BatchEngine::ensure_graph(const BatchPlan& pl, std::string& err) {
         hipGraphNode_t node = nullptr;
         e = hipGraphAddKernelNode(&node, graph_, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
         prev = node;
+        return node;
     };
     void* aReset[] = {&ctr_};
+    argTables_ = t;
     void* aStage[] = {&jobs_, &t};
     void* aPaths[] = {&jobs_, &items_, &t, &ctr_, &trash_};
     void* aPathsBig[] = {&jobs_, &itemsBig_, &t, &ctr_, &trash_};
     add(reinterpret_cast<const void*>(&reset_counters_kernel), 1, 64, 0, aReset);
-    add(reinterpret_cast<const void*>(&batch_cost_kernel), gCost, 256, costLds, aStage);
+    costNode_ = add(reinterpret_cast<const void*>(&batch_cost_kernel), 1, 256, costLds, aStage);
     add(reinterpret_cast<const void*>(&batch_paths_kernel<0>), gPaths, 256, 0, aPaths);
     if (withBig)
         add(reinterpret_cast<const void*>(&batch_paths_kernel<1>), gPathsBig, 256, 0, aPathsBig);
-    add(reinterpret_cast<const void*>(&batch_wta_kernel<0>), gWta0, 256, wta0Lds, aStage);
-    if (withBig) add(reinterpret_cast<const void*>(&batch_wta_kernel<1>), gWta1, 256, wta1Lds, aStage);
+    wtaNode_[0] = add(reinterpret_cast<const void*>(&batch_wta_kernel<0>), 1, 256, wta0Lds, aStage);
+    wtaNode_[1] = withBig ? add(reinterpret_cast<const void*>(&batch_wta_kernel<1>), 1, 256, wta1Lds,
+                                aStage)
+                          : nullptr;
     if (withSpk) add(reinterpret_cast<const void*>(&batch_speckle_kernel), gSpk, 256, 0, aStage);
     add(reinterpret_cast<const void*>(&batch_median_kernel), gMed, 256, 0, aStage);
     if (e != hipSuccess) return fail(e, "graph kernel node");
@@ -548,6 +535,27 @@ int BatchEngine::enqueue(BatchPlan& pl, std::string& err) {
     }
     int rc = ensure_graph(pl, err);
     if (rc) return rc;
+    // The one-block-per-unit stages get this batch's unit counts as their grids.
+    void* args[] = {&jobs_, &argTables_};
+    auto set_grid = [&](hipGraphNode_t node, const void* fn, int units, size_t lds) {
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void*>(fn);
+        kp.gridDim = dim3(static_cast<unsigned>(std::max(1, units)));
+        kp.blockDim = dim3(256);
+        kp.sharedMemBytes = static_cast<unsigned>(lds);
+        kp.kernelParams = args;
+        kp.extra = nullptr;
+        return hipGraphExecKernelNodeSetParams(exec_, node, &kp);
+    };
+    e = set_grid(costNode_, reinterpret_cast<const void*>(&batch_cost_kernel), pl.costStart.back(),
+                 capCostLds_);
+    if (e == hipSuccess)
+        e = set_grid(wtaNode_[0], reinterpret_cast<const void*>(&batch_wta_kernel<0>),
+                     pl.wta0Start.back(), capWta0Lds_);
+    if (e == hipSuccess && wtaNode_[1])
+        e = set_grid(wtaNode_[1], reinterpret_cast<const void*>(&batch_wta_kernel<1>),
+                     pl.wta1Start.back(), capWta1Lds_);
+    if (e != hipSuccess) return fail(e, "graph node update");
     e = hipGraphLaunch(exec_, s);
     if (e != hipSuccess) return fail(e, "graph launch");
     e = hipEventRecord(done_, s);
