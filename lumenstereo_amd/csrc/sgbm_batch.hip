@@ -341,16 +341,26 @@ void build_batch_plan(BatchPlan& pl) {
         }
     }
     // Longest scanlines first (by the wave's first scanline, the longest of its four except at
-    // the diagonal corner), so the stage does not end on a long wave started late.
-    auto byLen = [&](const PathItem& a, const PathItem& b) {
-        const PairJob& ja = pl.jobs[a.pair];
-        const PairJob& jb = pl.jobs[b.pair];
-        const int la = scan_length(a.path, a.s0, ja.W, ja.H) * ja.K;
-        const int lb = scan_length(b.path, b.s0, jb.W, jb.H) * jb.K;
-        return la > lb;
+    // the diagonal corner, times the slice width), so the stage does not end on a long wave
+    // started late. The key is a small integer (<= 4600 * 32): a stable counting sort.
+    auto by_length = [&](std::vector<PathItem>& v) {
+        if (v.size() < 2) return;
+        std::vector<int> key(v.size());
+        int maxKey = 0;
+        for (size_t i = 0; i < v.size(); ++i) {
+            const PairJob& jb = pl.jobs[v[i].pair];
+            key[i] = scan_length(v[i].path, v[i].s0, jb.W, jb.H) * jb.K;
+            maxKey = std::max(maxKey, key[i]);
+        }
+        std::vector<int> pos(maxKey + 2, 0);
+        for (int k : key) ++pos[maxKey - k + 1];
+        for (int k = 1; k <= maxKey + 1; ++k) pos[k] += pos[k - 1];
+        std::vector<PathItem> out(v.size());
+        for (size_t i = 0; i < v.size(); ++i) out[pos[maxKey - key[i]]++] = v[i];
+        v.swap(out);
     };
-    std::stable_sort(pl.items.begin(), pl.items.end(), byLen);
-    std::stable_sort(pl.itemsBig.begin(), pl.itemsBig.end(), byLen);
+    by_length(pl.items);
+    by_length(pl.itemsBig);
 }
 
 BatchEngine::~BatchEngine() {
