@@ -305,6 +305,77 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
     }
 
 
+def bench_scene(ls, torch, W, H, reps):
+    """The stages either side of the matcher on one W x H pair: the rectification remap of both
+    images (one launch), disparity -> depth, the colour map, and the fused depth + colour pass.
+    Each launch is bracketed by its own hipEvents on the stream it runs on; the figure is the
+    mean launch time over `reps` launches. Algorithmic bytes per pixel: remap 8 (map) + 1 (source
+    tap, the image is read once) + 1 (write) per image; depth 2 + 4; colour 4 + 3; fused 2 + 3."""
+    from lumenstereo_amd import scene
+
+    dev = torch.device("cuda:0")
+    left, right = ls.synth_pair(W, H, 4, 124, SEED + 9)
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    # A mild rectifying warp: small rotation and vertical shear about the image centre.
+    cx, cy = W / 2.0, H / 2.0
+    m0x = (cx + (xs - cx) * np.float32(0.9995) - (ys - cy) * np.float32(0.01)).astype(np.float32)
+    m0y = (cy + (ys - cy) * np.float32(0.9995) + (xs - cx) * np.float32(0.01)).astype(np.float32)
+    m1x = (xs + np.float32(0.37)).astype(np.float32)
+    m1y = (ys - np.float32(0.61) + (xs - cx) * np.float32(0.002)).astype(np.float32)
+    r = scene.StereoRectifier(scene.CameraParams(width=W, height=H, map0x=m0x, map0y=m0y,
+                                                 map1x=m1x, map1y=m1y))
+    dl, dr = torch.from_numpy(left).to(dev), torch.from_numpy(right).to(dev)
+    ol, orr = torch.empty_like(dl), torch.empty_like(dr)
+    r.rectify_device(dl.data_ptr(), dr.data_ptr(), W, H, 1, ol.data_ptr(), orr.data_ptr())
+    rect_ms = 0.0
+    for _ in range(reps):
+        r.rectify_device(dl.data_ptr(), dr.data_ptr(), W, H, 1, ol.data_ptr(), orr.data_ptr(),
+                         sync=False)
+        rect_ms += r.last_time_ms()
+    rect_ms /= reps
+
+    m = ls.StereoSGBM(workload_params(ls, 128, True, 8))
+    d_disp = torch.empty((H, W), dtype=torch.int16, device=dev)
+    m.compute_device(ol.data_ptr(), orr.data_ptr(), W, H, d_disp.data_ptr(), sync=True)
+    dz = torch.empty((H, W), dtype=torch.float32, device=dev)
+    dc = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
+    f, B, lo, hi = 1400.0, 0.12, 0.5, 20.0
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    launches = {
+        "depth": lambda: scene.disparityToDepthGPU(d_disp.data_ptr(), dz.data_ptr(), W, H, f, B,
+                                                   0.0, s),
+        "colour_map": lambda: scene.colorizeDepthGPU(dz.data_ptr(), dc.data_ptr(), W, H, lo, hi, s),
+        "depth_colour_fused": lambda: scene.depthColorizeGPU(d_disp.data_ptr(), None,
+                                                             dc.data_ptr(), W, H, f, B, 0.0, lo,
+                                                             hi, s),
+    }
+    times = {}
+    for name, fn in launches.items():
+        fn()
+        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+               for _ in range(reps)]
+        for a, b in evs:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        times[name] = sum(a.elapsed_time(b) for a, b in evs) / reps
+    npx = W * H
+    per_px = {"rectify_pair": 20, "depth": 6, "colour_map": 7, "depth_colour_fused": 5}
+    times["rectify_pair"] = rect_ms
+    out = {"workload": f"{W}x{H} grey pair: rectify both images (one launch), then the "
+                       f"{W}x{H} D=128 Census 8-path disparity -> depth -> colour",
+           "timing": f"mean of {reps} launches, hipEvents around each on its stream"}
+    for name in ("rectify_pair", "depth", "colour_map", "depth_colour_fused"):
+        ms = times[name]
+        gbs = per_px[name] * npx / (ms * 1e-3) / 1e9
+        out[name] = {"ms": round(ms, 4), "bytes_per_pixel": per_px[name],
+                     "GB_per_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    del r, m
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +387,10 @@ def main() -> int:
     ap.add_argument("--skip-forms", action="store_true",
                     help="only the single-pair workload (no batched / serving measurement)")
     ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--skip-scene", action="store_true",
+                    help="no measurement of the rectification / depth / colour stages")
+    ap.add_argument("--scene-size", default="2880x1988",
+                    help="image size of the rectification / depth / colour measurement")
     ap.add_argument("--mix-pairs", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--serve-rates", default="20,50,100,200,300,400")
@@ -416,6 +491,10 @@ def main() -> int:
         cb, mism = cpu_baseline(ls, params, left, right, gpu_out, args.cpu_threads, desc)
         line["cpu_baseline"] = cb
         line["parity_vs_oracle_mismatched_pixels"] = mism
+    if not args.skip_scene:
+        sw, sh = (int(v) for v in args.scene_size.lower().split("x"))
+        print(f"# scene stages {sw}x{sh}", file=sys.stderr, flush=True)
+        line["scene"] = bench_scene(ls, torch, sw, sh, 20)
     if not args.skip_forms:
         del m, d_left, d_right, d_out
         gc.collect()

@@ -240,6 +240,34 @@ _SIGNATURES = {
     "lumen_server_drain": (_c_int, [_c_void_p]),
     "lumen_server_get_stats": (_c_int, [_c_void_p, ctypes.POINTER(CServerStats)]),
     "lumen_server_last_error": (ctypes.c_char_p, [_c_void_p]),
+    # include/lumen_rectify.h
+    "lumen_rectifier_create": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int,
+                                        _c_int, ctypes.POINTER(_c_void_p)]),
+    "lumen_rectifier_destroy": (_c_int, [_c_void_p]),
+    "lumen_rectifier_is_pre_rectified": (_c_int, [_c_void_p]),
+    "lumen_rectifier_rectify": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int,
+                                         _c_void_p, _c_void_p]),
+    "lumen_rectifier_rectify_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
+                                                _c_int, _c_void_p, _c_void_p, _c_int]),
+    "lumen_rectifier_get_stream": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
+    "lumen_rectifier_get_time_ms": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "lumen_rectifier_last_error": (ctypes.c_char_p, [_c_void_p]),
+    "lumen_remap": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int, _c_int, _c_int]),
+    "lumen_remap_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
+                                    _c_int, _c_void_p]),
+    # include/lumen_depth.h
+    "lumen_disparity_to_depth_device": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int,
+                                                 ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                                 _c_void_p]),
+    "lumen_disparity_to_depth": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double]),
+    "lumen_colorize_depth_device": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, ctypes.c_float,
+                                             ctypes.c_float, _c_void_p]),
+    "lumen_colorize_depth": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int, ctypes.c_float,
+                                      ctypes.c_float]),
+    "lumen_depth_colorize_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
+                                             ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_float, _c_void_p]),
 }
 
 

@@ -21,7 +21,7 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_paths.hip", "sgbm_capi.hip", "sgbm_batch.hip",
-               "sgbm_serve.hip", "sgbm_tune.hip"]
+               "sgbm_serve.hip", "sgbm_tune.hip", "rectify.hip", "depth_map.hip"]
 CXX_SOURCES = ["synth_scene.cpp", "host_copy.cpp"]
 
 HIP_FLAGS = [
@@ -64,7 +64,7 @@ def build_library() -> Path:
 def build_oracle() -> Path:
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-pthread",
-          ORACLE_DIR / "sgbm_oracle.cpp", "-o", tmp])
+          ORACLE_DIR / "sgbm_oracle.cpp", ORACLE_DIR / "scene_oracle.cpp", "-o", tmp])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB
 

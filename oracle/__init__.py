@@ -3,7 +3,9 @@
 TEST INFRASTRUCTURE, NOT THE PRODUCT: only tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg import this module, and only as the checker (or as the timed CPU baseline).
 The oracle restates the reference kernels of Pfactorial01/LumenStereo (src/sgbm_kernels.cu,
-src/sgbm_gpu.cu) single-threaded; see the header of sgbm_oracle.h for what pins each function.
+src/sgbm_gpu.cu; src/rectification.cu and src/depth_map.cu for the stages either side of the
+matcher) single-threaded; see the headers of sgbm_oracle.h and scene_oracle.h for what pins
+each function.
 Parity for the speckle stage is unpinned (the reference kernel races).
 """
 from __future__ import annotations
@@ -55,15 +57,25 @@ def lib() -> ctypes.CDLL:
             "oracle_sgbm": [ctypes.POINTER(OracleParams), vp, vp, ci, ci, vp,
                             ctypes.POINTER(OracleDumps)],
             "oracle_sgbm_mt": [ctypes.POINTER(OracleParams), vp, vp, ci, ci, vp, ci],
+            # scene_oracle.h
+            "oracle_remap": [vp, vp, vp, vp, ci, ci, ci],
+            "oracle_disparity_to_depth": [vp, vp, ci, ci, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_float],
+            "oracle_disparity_to_depth_host": [vp, vp, ci, ci, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double],
+            "oracle_colorize_depth": [vp, vp, ci, ci, ctypes.c_float, ctypes.c_float],
+            "oracle_turbo_entry": [ci],
         }
+        void_fns = ("oracle_prefilter", "oracle_census", "oracle_cost_sad", "oracle_cost_census",
+                    "oracle_aggregate_path", "oracle_wta_left", "oracle_wta_right",
+                    "oracle_lr_check", "oracle_speckle", "oracle_median", "oracle_remap",
+                    "oracle_disparity_to_depth", "oracle_disparity_to_depth_host",
+                    "oracle_colorize_depth")
         for name, args in sigs.items():
             f = getattr(l, name)
             f.argtypes = args
-            f.restype = ci if name not in ("oracle_prefilter", "oracle_census", "oracle_cost_sad",
-                                           "oracle_cost_census", "oracle_aggregate_path",
-                                           "oracle_wta_left", "oracle_wta_right",
-                                           "oracle_lr_check", "oracle_speckle",
-                                           "oracle_median") else None
+            f.restype = (None if name in void_fns else
+                         ctypes.c_uint32 if name == "oracle_turbo_entry" else ci)
         _lib = l
     return _lib
 
@@ -199,3 +211,47 @@ def sgbm_mt(params, left, right, nthreads):
     if rc != 0:
         raise ValueError("oracle_sgbm_mt rejected the arguments")
     return out
+
+
+def remap(src, mapx, mapy):
+    """src/rectification.cu:201-256: src H x W (or H x W x C) uint8, float32 maps H x W."""
+    src = _c(src, np.uint8)
+    H, W = src.shape[:2]
+    C = 1 if src.ndim == 2 else src.shape[2]
+    mapx, mapy = _c(mapx, np.float32), _c(mapy, np.float32)
+    assert mapx.shape == (H, W) and mapy.shape == (H, W)
+    out = np.empty_like(src)
+    lib().oracle_remap(_p(src), _p(mapx), _p(mapy), _p(out), W, H, C)
+    return out
+
+
+def disparity_to_depth(disp, f, B, doffs=0.0):
+    """src/depth_map.cu:74-101 (the GPU kernel's float arithmetic)."""
+    disp = _c(disp, np.int16)
+    out = np.empty(disp.shape, np.float32)
+    lib().oracle_disparity_to_depth(_p(disp), _p(out), disp.shape[1], disp.shape[0], f, B, doffs)
+    return out
+
+
+def disparity_to_depth_host(disp, f, B, doffs=0.0):
+    """src/depth_map.cu:142-160 (the host function's double quotient)."""
+    disp = _c(disp, np.int16)
+    out = np.empty(disp.shape, np.float32)
+    lib().oracle_disparity_to_depth_host(_p(disp), _p(out), disp.shape[1], disp.shape[0], f, B,
+                                         doffs)
+    return out
+
+
+def colorize_depth(depth, min_depth, max_depth):
+    """src/depth_map.cu:106-138: H x W float32 -> H x W x 3 uint8 (R, G, B)."""
+    depth = _c(depth, np.float32)
+    out = np.empty(depth.shape + (3,), np.uint8)
+    lib().oracle_colorize_depth(_p(depth), _p(out), depth.shape[1], depth.shape[0], min_depth,
+                                max_depth)
+    return out
+
+
+def turbo_table():
+    """The oracle's colour table as a 256 x 3 uint8 array (R, G, B)."""
+    v = np.array([lib().oracle_turbo_entry(i) for i in range(256)], np.uint32)
+    return np.stack([v & 0xFF, (v >> 8) & 0xFF, (v >> 16) & 0xFF], axis=1).astype(np.uint8)

@@ -1,6 +1,6 @@
 """Host-side checks of the C ABI and its Python mirror (no GPU work).
 
-Covers: the library loads and exports every function include/lumen_sgbm.h declares; defaults
+Covers: the library loads and exports every function include/lumen_*.h declares; defaults
 and helpers mirror the reference's StereoParams (tests/test_sgbm.cpp:28-75); validation errors
 are raised before any device call with the reference's messages (src/sgbm_gpu.cu:148-166);
 footprint arithmetic; the synthetic scene generator is deterministic.
@@ -14,20 +14,20 @@ import pytest
 
 import lumenstereo_amd as ls
 
-HEADER = Path(__file__).resolve().parent.parent / "include" / "lumen_sgbm.h"
+HEADERS = sorted((Path(__file__).resolve().parent.parent / "include").glob("lumen_*.h"))
 
 
 def declared_functions():
-    text = HEADER.read_text()
+    text = "".join(h.read_text() for h in HEADERS)
     return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(lumen_\w+)\s*\(", text, re.M)))
 
 
 def test_header_symbols_exported():
     names = declared_functions()
-    assert len(names) >= 15
+    assert len(HEADERS) >= 3 and len(names) >= 45
     lib = ctypes.CDLL(str(ls.LIB_PATH))
     for n in names:
-        assert hasattr(lib, n), f"{n} declared in lumen_sgbm.h but not exported"
+        assert hasattr(lib, n), f"{n} declared in include/ but not exported"
     # and every declared symbol has a Python signature
     assert set(names) <= set(ls._SIGNATURES), set(names) - set(ls._SIGNATURES)
 

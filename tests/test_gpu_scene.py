@@ -1,0 +1,201 @@
+"""Parity of the HIP rectification remap, disparity -> depth and colour map (through the C ABI)
+against the CPU oracle: bit-exact, since both evaluate the same single-precision operations in
+the same order without contraction. Mirrors tests/test_rectification.cpp:160-265 of the
+reference (pre-rectified passthrough, identity map, half-pixel shift) and adds random maps,
+multi-channel images, odd sizes (partial 4-pixel groups) and unaligned buffers (the one-pixel
+kernels)."""
+import numpy as np
+import pytest
+
+import lumenstereo_amd as ls
+from lumenstereo_amd import scene
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+rng = np.random.default_rng(77)
+
+
+def _maps(H, W, lo=-2.0, spread=3.0):
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    mx = (xs + rng.uniform(lo, lo + spread, (H, W))).astype(np.float32)
+    my = (ys + rng.uniform(lo, lo + spread, (H, W))).astype(np.float32)
+    mx[::17, ::13] = np.nan
+    my[5::19, 3::11] = 4e9
+    return mx, my
+
+
+@pytest.mark.parametrize("shape", [(48, 64, 1), (37, 53, 1), (33, 41, 3), (20, 30, 4), (1, 7, 1)])
+def test_remap_host_form_matches_oracle(gpu_available, shape):
+    H, W, C = shape
+    src = rng.integers(0, 256, (H, W) if C == 1 else (H, W, C), dtype=np.uint8)
+    mx, my = _maps(H, W)
+    got = scene.rectifyImageGPU(src, mx, my)
+    assert np.array_equal(got, oracle.remap(src, mx, my))
+
+
+def test_remap_identity_and_half_shift(gpu_available):
+    """tests/test_rectification.cpp:189-265: identity exact; half-pixel shift within 1 of
+    cv::remap (here: equal to the closed form, which the oracle tests pin)."""
+    H, W = 480, 640
+    src = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    assert np.array_equal(scene.rectifyImageGPU(src, xs, ys), src)
+    g = np.tile((np.arange(100) * 255 // 100).astype(np.uint8), (100, 1))
+    ys, xs = np.mgrid[0:100, 0:100].astype(np.float32)
+    assert np.array_equal(scene.rectifyImageGPU(g, xs + 0.5, ys + 0.5),
+                          oracle.remap(g, xs + 0.5, ys + 0.5))
+
+
+def test_remap_device_unaligned_and_tail(gpu_available):
+    import torch
+
+    dev = torch.device("cuda:0")
+    H, W = 31, 45  # 1395 pixels: not a multiple of 4
+    src = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    mx, my = _maps(H, W)
+    want = oracle.remap(src, mx, my)
+    for off in (0, 1):  # off = 1 misaligns maps and output: the one-pixel kernel runs
+        d_src = torch.from_numpy(src).to(dev)
+        d_mx = torch.zeros(H * W + 1, dtype=torch.float32, device=dev)
+        d_my = torch.zeros_like(d_mx)
+        d_mx[off:off + H * W] = torch.from_numpy(mx.ravel()).to(dev)
+        d_my[off:off + H * W] = torch.from_numpy(my.ravel()).to(dev)
+        d_out = torch.zeros(H * W + 1, dtype=torch.uint8, device=dev)
+        scene.remapDevice(d_src.data_ptr(), d_mx.data_ptr() + 4 * off, d_my.data_ptr() + 4 * off,
+                          d_out.data_ptr() + off, W, H, 1, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out[off:off + H * W].cpu().numpy().reshape(H, W), want), off
+
+
+def test_rectifier_pair_and_passthrough(gpu_available):
+    H, W = 120, 160
+    left = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    right = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    m0x, m0y = _maps(H, W)
+    m1x, m1y = _maps(H, W, lo=-1.0, spread=2.0)
+    cp = scene.CameraParams(width=W, height=H, map0x=m0x, map0y=m0y, map1x=m1x, map1y=m1y)
+    r = scene.StereoRectifier(cp)
+    assert not r.isPreRectified()
+    lo, ro = r.rectify(left, right)
+    assert np.array_equal(lo, oracle.remap(left, m0x, m0y))
+    assert np.array_equal(ro, oracle.remap(right, m1x, m1y))
+    assert r.last_time_ms() >= 0
+    with pytest.raises(ls.StereoException):
+        r.rectify(left[:, :-1], right[:, :-1])  # size differs from the maps
+    # tests/test_rectification.cpp:160-187: no maps -> pre-rectified, output equals input.
+    pre = scene.StereoRectifier(scene.CameraParams(width=640, height=480))
+    assert pre.isPreRectified()
+    l3 = rng.integers(0, 256, (480, 640, 3), dtype=np.uint8)
+    r3 = rng.integers(0, 256, (480, 640, 3), dtype=np.uint8)
+    a, b = pre.rectify(l3, r3)
+    assert np.array_equal(a, l3) and np.array_equal(b, r3)
+
+
+def test_rectifier_device_form(gpu_available):
+    import torch
+
+    dev = torch.device("cuda:0")
+    H, W = 96, 128
+    m0x, m0y = _maps(H, W)
+    m1x, m1y = _maps(H, W)
+    r = scene.StereoRectifier(scene.CameraParams(width=W, height=H, map0x=m0x, map0y=m0y,
+                                                 map1x=m1x, map1y=m1y))
+    left = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    right = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    dl, dr = torch.from_numpy(left).to(dev), torch.from_numpy(right).to(dev)
+    ol, orr = torch.empty_like(dl), torch.empty_like(dr)
+    torch.cuda.synchronize()
+    r.rectify_device(dl.data_ptr(), dr.data_ptr(), W, H, 1, ol.data_ptr(), orr.data_ptr())
+    assert np.array_equal(ol.cpu().numpy(), oracle.remap(left, m0x, m0y))
+    assert np.array_equal(orr.cpu().numpy(), oracle.remap(right, m1x, m1y))
+
+
+def _disparity(H, W):
+    d = rng.integers(-40, 4000, (H, W)).astype(np.int16)
+    d[::9, ::7] = -1
+    d[3::11, ::5] = 0
+    d[0, :4] = [32767, -32768, 1, 16]
+    return d
+
+
+@pytest.mark.parametrize("shape", [(120, 160), (37, 53)])
+def test_depth_and_colour_match_oracle(gpu_available, shape):
+    import torch
+
+    dev = torch.device("cuda:0")
+    H, W = shape
+    d = _disparity(H, W)
+    f, B, off, lo, hi = 4161.221, 0.176252, 2.5, 0.5, 20.0
+    dd = torch.from_numpy(d).to(dev)
+    dz = torch.empty((H, W), dtype=torch.float32, device=dev)
+    dc = torch.empty((H, W, 3), dtype=torch.uint8, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    scene.disparityToDepthGPU(dd.data_ptr(), dz.data_ptr(), W, H, f, B, off, s)
+    scene.colorizeDepthGPU(dz.data_ptr(), dc.data_ptr(), W, H, lo, hi, s)
+    torch.cuda.synchronize()
+    z_want = oracle.disparity_to_depth(d, f, B, off)
+    z_got = dz.cpu().numpy()
+    assert np.array_equal(z_got.view(np.uint32), z_want.view(np.uint32))
+    c_want = oracle.colorize_depth(z_want, lo, hi)
+    assert np.array_equal(dc.cpu().numpy(), c_want)
+    # Fused form: the same depth and colours, with and without the depth output.
+    dz2 = torch.full((H, W), -7.0, dtype=torch.float32, device=dev)
+    dc2 = torch.zeros((H, W, 3), dtype=torch.uint8, device=dev)
+    scene.depthColorizeGPU(dd.data_ptr(), dz2.data_ptr(), dc2.data_ptr(), W, H, f, B, off, lo, hi, s)
+    dc3 = torch.zeros((H, W, 3), dtype=torch.uint8, device=dev)
+    scene.depthColorizeGPU(dd.data_ptr(), None, dc3.data_ptr(), W, H, f, B, off, lo, hi, s)
+    torch.cuda.synchronize()
+    assert np.array_equal(dz2.cpu().numpy().view(np.uint32), z_want.view(np.uint32))
+    assert np.array_equal(dc2.cpu().numpy(), c_want)
+    assert np.array_equal(dc3.cpu().numpy(), c_want)
+
+
+def test_depth_host_forms_match_oracle(gpu_available):
+    H, W = 50, 77
+    d = _disparity(H, W)
+    z = scene.disparityToDepth(d, 4161.221, 0.176252, 209.059)
+    want = oracle.disparity_to_depth_host(d, 4161.221, 0.176252, 209.059)
+    assert np.array_equal(z.view(np.uint32), want.view(np.uint32))
+    zz = rng.uniform(-1, 25, (H, W)).astype(np.float32)
+    zz[::6, ::4] = np.nan
+    assert np.array_equal(scene.colorizeDepth(zz, 1.0, 10.0), oracle.colorize_depth(zz, 1.0, 10.0))
+
+
+def test_depth_unaligned_buffers(gpu_available):
+    import torch
+
+    dev = torch.device("cuda:0")
+    H, W = 19, 23
+    d = _disparity(H, W)
+    n = H * W
+    dd = torch.zeros(n + 1, dtype=torch.int16, device=dev)
+    dd[1:] = torch.from_numpy(d.ravel()).to(dev)
+    dz = torch.zeros(n + 1, dtype=torch.float32, device=dev)
+    dc = torch.zeros(3 * n + 1, dtype=torch.uint8, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    scene.disparityToDepthGPU(dd.data_ptr() + 2, dz.data_ptr() + 4, W, H, 1000.0, 0.1, 0.0, s)
+    scene.colorizeDepthGPU(dz.data_ptr() + 4, dc.data_ptr() + 1, W, H, 1.0, 10.0, s)
+    torch.cuda.synchronize()
+    z_want = oracle.disparity_to_depth(d, 1000.0, 0.1, 0.0)
+    assert np.array_equal(dz[1:].cpu().numpy().reshape(H, W), z_want)
+    assert np.array_equal(dc[1:].cpu().numpy().reshape(H, W, 3), oracle.colorize_depth(z_want, 1.0, 10.0))
+
+
+def test_sgbm_to_colour_pipeline(gpu_available):
+    """Rectify (identity maps), match, depth and colour in sequence, each against the oracle."""
+    H, W = 90, 128
+    left, right = ls.synth_pair(W, H, 4, 28, seed=5)
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    r = scene.StereoRectifier(scene.CameraParams(width=W, height=H, map0x=xs, map0y=ys,
+                                                 map1x=xs + 0.25, map1y=ys))
+    lr, rr = r.rectify(left, right)
+    assert np.array_equal(lr, left)
+    assert np.array_equal(rr, oracle.remap(right, xs + 0.25, ys))
+    p = ls.StereoParams(maxDisparity=32, numDirections=8, matchingCostMode=ls.MatchingCostMode.Census,
+                        uniquenessRatio=0, speckleWindowSize=0)
+    disp = ls.StereoSGBM(p).compute(lr, rr)
+    assert np.array_equal(disp, oracle.sgbm(p, lr, rr))
+    z = scene.disparityToDepth(disp, 500.0, 0.12, 0.0)
+    assert np.array_equal(z, oracle.disparity_to_depth_host(disp, 500.0, 0.12, 0.0))
+    assert np.array_equal(scene.colorizeDepth(z, 0.5, 20.0), oracle.colorize_depth(z, 0.5, 20.0))
