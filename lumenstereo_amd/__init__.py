@@ -194,9 +194,20 @@ class CServerStats(ctypes.Structure):
         "pool_bytes", "pool_high_water", "max_waiting")]
 
 
+class CCameraParams(ctypes.Structure):
+    """lumen_camera_params (include/lumen_config.h)."""
+
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("K0", ctypes.c_double * 9), ("D0", ctypes.c_double * 5),
+                ("K1", ctypes.c_double * 9), ("D1", ctypes.c_double * 5),
+                ("R", ctypes.c_double * 9), ("T", ctypes.c_double * 3),
+                ("baseline", ctypes.c_double), ("focalLength", ctypes.c_double),
+                ("disparityOffset", ctypes.c_int32), ("numDisparities", ctypes.c_int32)]
+
+
 _lib = None
 
-# Every symbol include/lumen_sgbm.h declares, with its ctypes signature.
+# Every symbol include/lumen_*.h declares, with its ctypes signature.
 _c_int, _c_void_p, _c_size_t = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
 _SIGNATURES = {
     "lumen_sgbm_default_params": (_c_int, [ctypes.POINTER(CParams)]),
@@ -240,6 +251,10 @@ _SIGNATURES = {
     "lumen_server_drain": (_c_int, [_c_void_p]),
     "lumen_server_get_stats": (_c_int, [_c_void_p, ctypes.POINTER(CServerStats)]),
     "lumen_server_last_error": (ctypes.c_char_p, [_c_void_p]),
+    # include/lumen_config.h
+    "lumen_params_load_yaml": (_c_int, [ctypes.c_char_p, ctypes.POINTER(CParams)]),
+    "lumen_params_save_yaml": (_c_int, [ctypes.c_char_p, ctypes.POINTER(CParams)]),
+    "lumen_middlebury_calib_load": (_c_int, [ctypes.c_char_p, ctypes.POINTER(CCameraParams)]),
     # include/lumen_rectify.h
     "lumen_rectifier_create": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int,
                                         _c_int, ctypes.POINTER(_c_void_p)]),

@@ -1,5 +1,7 @@
 // C++ API tests mirroring the reference's tests/test_sgbm.cpp (StereoParams, validation and
-// exception cases, output shape/type on GPU). gtest is not available, so this is a minimal
+// exception cases, output shape/type on GPU, depth) and tests/test_rectification.cpp
+// (CameraParams, parameter files, Middlebury calibration, rectification). gtest is not
+// available, so this is a minimal
 // runner: `test_stereo_api` runs the host-only cases, `test_stereo_api --gpu` adds the cases
 // that compute on the GPU. Exit status 0 = all passed.
 #include <cstdio>
@@ -8,6 +10,11 @@
 #include <string>
 #include <vector>
 
+#include <cmath>
+#include <fstream>
+
+#include "stereo/depth_map.h"
+#include "stereo/rectification.h"
 #include "stereo/sgbm_gpu.h"
 
 using namespace stereo;
@@ -201,6 +208,139 @@ static void DifferentSizesAndDisparities() {
     EXPECT(true);
 }
 
+// --- tests/test_rectification.cpp:21-145 (host) ------------------------------------------
+static void CameraDefaultInvalid() {
+    CameraParams p;
+    EXPECT(!p.isValid());
+}
+static void CameraComputeDerivedValues() {
+    CameraParams p;
+    p.width = 640;
+    p.height = 480;
+    p.K0 = {500, 0, 320, 0, 500, 240, 0, 0, 1};
+    p.K1 = p.K0;
+    p.R = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    p.T = {-0.1, 0, 0};
+    p.D0.assign(5, 0.0);
+    p.D1.assign(5, 0.0);
+    p.computeDerivedValues();
+    EXPECT(std::fabs(p.baseline - 0.1) < 1e-6);
+    EXPECT(std::fabs(p.focalLength - 500.0) < 1e-6);
+    EXPECT(p.isValid());
+}
+static void CalibNonexistentFile() {
+    CameraParams p;
+    EXPECT(!loadMiddleburyCalib("/nonexistent/path/calib.txt", p));
+}
+static void CalibMiddleburyFormat() {
+    const std::string path = "/tmp/lumen_test_calib.txt";
+    {
+        std::ofstream f(path);
+        f << "cam0=[4161.221 0 1445.577; 0 4161.221 984.686; 0 0 1]\n"
+             "cam1=[4161.221 0 1654.636; 0 4161.221 984.686; 0 0 1]\n"
+             "doffs=209.059\nbaseline=176.252\nwidth=2880\nheight=1988\nndisp=280\n";
+    }
+    CameraParams p;
+    EXPECT(loadMiddleburyCalib(path, p));
+    EXPECT(p.width == 2880 && p.height == 1988);
+    EXPECT(p.focalLength == 4161.221);
+    EXPECT(std::fabs(p.baseline - 0.176252) < 1e-12);
+    EXPECT(p.disparityOffset == 209 && p.numDisparities == 280);
+    EXPECT(p.isValid());
+    std::remove(path.c_str());
+}
+static void StereoParamsSaveLoad() {
+    StereoParams original;
+    original.minDisparity = 10;
+    original.maxDisparity = 200;
+    original.blockSize = 7;
+    original.P1 = 15;
+    original.P2 = 60;
+    const std::string path = "/tmp/lumen_test_stereo_params.yaml";
+    EXPECT(saveStereoParams(path, original));
+    StereoParams loaded;
+    EXPECT(loadStereoParams(path, loaded));
+    EXPECT(loaded.minDisparity == 10 && loaded.maxDisparity == 200 && loaded.blockSize == 7);
+    EXPECT(loaded.P1 == 15 && loaded.P2 == 60);
+    std::remove(path.c_str());
+}
+static void StereoParamsMatchingCostRoundtrip() {
+    StereoParams original;
+    original.matchingCostMode = MatchingCostMode::Census;
+    const std::string path = "/tmp/lumen_test_stereo_params_census.yaml";
+    EXPECT(saveStereoParams(path, original));
+    StereoParams loaded;
+    EXPECT(loadStereoParams(path, loaded));
+    EXPECT(loaded.matchingCostMode == MatchingCostMode::Census);
+    std::remove(path.c_str());
+}
+// tests/test_sgbm.cpp:320-332.
+static void DepthFormula() {
+    const float focalLength = 1000.0f, baseline = 0.1f, disparity = 100.0f;
+    EXPECT(focalLength * baseline / disparity == 1.0f);
+}
+
+// --- tests/test_rectification.cpp:160-265, tests/test_sgbm.cpp:320-349 (GPU) --------------
+static void PreRectifiedPassthrough() {
+    CameraParams p;
+    p.width = 640;
+    p.height = 480;
+    StereoRectifier r(p);
+    EXPECT(r.isPreRectified());
+    std::vector<uint8_t> l(640 * 480 * 3), rr(640 * 480 * 3), lo(l.size()), ro(l.size());
+    std::srand(3);
+    for (auto& x : l) x = std::rand() & 255;
+    for (auto& x : rr) x = std::rand() & 255;
+    r.rectify(l.data(), rr.data(), 640, 480, 3, lo.data(), ro.data());
+    EXPECT(lo == l && ro == rr);
+}
+static void IdentityMapExact() {
+    const int W = 640, H = 480;
+    std::vector<uint8_t> src(W * H), dst(W * H);
+    for (auto& x : src) x = std::rand() & 255;
+    std::vector<float> mx(W * H), my(W * H);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            mx[y * W + x] = static_cast<float>(x);
+            my[y * W + x] = static_cast<float>(y);
+        }
+    rectifyImageGPU(src.data(), mx.data(), my.data(), dst.data(), W, H);
+    EXPECT(dst == src);
+}
+static void BilinearHalfPixel() {
+    const int W = 100, H = 100;
+    std::vector<uint8_t> src(W * H), dst(W * H);
+    std::vector<float> mx(W * H), my(W * H);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            src[y * W + x] = static_cast<uint8_t>(x * 255 / W);
+            mx[y * W + x] = x + 0.5f;
+            my[y * W + x] = y + 0.5f;
+        }
+    rectifyImageGPU(src.data(), mx.data(), my.data(), dst.data(), W, H);
+    int maxDiff = 0;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            int s = 0;  // taps outside the image count as 0
+            for (int dy = 0; dy < 2; ++dy)
+                for (int dx = 0; dx < 2; ++dx)
+                    if (x + dx < W && y + dy < H) s += src[(y + dy) * W + x + dx];
+            maxDiff = std::max(maxDiff, std::abs(dst[y * W + x] - (s + 2) / 4));
+        }
+    EXPECT(maxDiff == 0);
+}
+static void DepthAndColorize() {
+    const int W = 100, H = 100;
+    std::vector<int16_t> d(W * H);
+    for (int i = 0; i < W * H; ++i) d[i] = static_cast<int16_t>(16 * (1 + i % 200));
+    std::vector<float> z(W * H);
+    disparityToDepth(d.data(), z.data(), W, H, 1000.0, 0.1);
+    EXPECT(z[99] == 1.0f);  // 100 px -> 1 m
+    std::vector<uint8_t> rgb(W * H * 3);
+    colorizeDepth(z.data(), rgb.data(), W, H, 1.0f, 10.0f);
+    EXPECT(rgb.size() == static_cast<size_t>(W * H * 3));
+}
+
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::string(argv[1]) == "--gpu";
     std::vector<std::pair<const char*, std::function<void()>>> tests = {
@@ -214,11 +354,22 @@ int main(int argc, char** argv) {
         {"SGBMTest.SizeMismatch", SizeMismatch},
         {"SGBMTest.DisparityRangeExceedsGpuLimit", DisparityRangeExceedsGpuLimit},
         {"SGBMTest.InvalidRangeMessage", InvalidRangeMessage},
+        {"CameraParamsTest.DefaultInvalid", CameraDefaultInvalid},
+        {"CameraParamsTest.ComputeDerivedValues", CameraComputeDerivedValues},
+        {"CalibrationLoadTest.NonexistentFile", CalibNonexistentFile},
+        {"CalibrationLoadTest.MiddleburyFormat", CalibMiddleburyFormat},
+        {"ConfigTest.StereoParamsSaveLoad", StereoParamsSaveLoad},
+        {"ConfigTest.StereoParamsMatchingCostRoundtrip", StereoParamsMatchingCostRoundtrip},
+        {"DepthMapTest.DisparityToDepth", DepthFormula},
     };
     if (gpu) {
         tests.push_back({"SGBMTest.ComputeBasic", ComputeBasic});
         tests.push_back({"SGBMTest.ColorInput", ColorInput});
         tests.push_back({"SGBMTest.DifferentSizesAndDisparities", DifferentSizesAndDisparities});
+        tests.push_back({"RectificationTest.PreRectifiedPassthrough", PreRectifiedPassthrough});
+        tests.push_back({"RectificationTest.GPUvsOpenCV", IdentityMapExact});
+        tests.push_back({"RectificationTest.BilinearInterpolation", BilinearHalfPixel});
+        tests.push_back({"DepthMapTest.ColorizeDepth", DepthAndColorize});
     }
     for (auto& t : tests) {
         const int before = g_failed;
