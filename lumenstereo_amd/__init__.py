@@ -205,6 +205,15 @@ class CCameraParams(ctypes.Structure):
                 ("disparityOffset", ctypes.c_int32), ("numDisparities", ctypes.c_int32)]
 
 
+class CEvalResult(ctypes.Structure):
+    """lumen_eval_result (include/lumen_evaluate.h)."""
+
+    _fields_ = [("valid", ctypes.c_int64), ("valid_pct", ctypes.c_double),
+                ("mae", ctypes.c_double), ("rms", ctypes.c_double), ("bad1", ctypes.c_double),
+                ("bad2", ctypes.c_double), ("bad4", ctypes.c_double), ("p50", ctypes.c_float),
+                ("p90", ctypes.c_float), ("p95", ctypes.c_float), ("p99", ctypes.c_float)]
+
+
 _lib = None
 
 # Every symbol include/lumen_*.h declares, with its ctypes signature.
@@ -255,6 +264,12 @@ _SIGNATURES = {
     "lumen_params_load_yaml": (_c_int, [ctypes.c_char_p, ctypes.POINTER(CParams)]),
     "lumen_params_save_yaml": (_c_int, [ctypes.c_char_p, ctypes.POINTER(CParams)]),
     "lumen_middlebury_calib_load": (_c_int, [ctypes.c_char_p, ctypes.POINTER(CCameraParams)]),
+    # include/lumen_evaluate.h
+    "lumen_pfm_load": (_c_int, [ctypes.c_char_p, ctypes.POINTER(_c_int), ctypes.POINTER(_c_int),
+                                _c_void_p, _c_size_t]),
+    "lumen_pfm_save": (_c_int, [ctypes.c_char_p, _c_void_p, _c_int, _c_int]),
+    "lumen_evaluate_disparity": (_c_int, [_c_void_p, _c_void_p, _c_int, _c_int,
+                                          ctypes.POINTER(CEvalResult)]),
     # include/lumen_rectify.h
     "lumen_rectifier_create": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p, _c_int,
                                         _c_int, ctypes.POINTER(_c_void_p)]),
