@@ -483,6 +483,23 @@ int BatchEngine::enqueue(BatchPlan& pl, std::string& err) {
         if (e != hipSuccess) return fail(e, "previous batch");
         pending_ = false;
     }
+    if (npairs == 1) {
+        // A batch of one pair runs the per-pair kernels: their grids cover exactly that pair,
+        // with its tuned schedule and a WTA specialised for its path count, where the
+        // persistent kernels pay for work lists that one pair does not need (the serving
+        // loop's batches at low load are single requests). The stage bodies are the same, so
+        // the output is too.
+        const PairJob& j = pl.jobs[0];
+        hipError_t e = launch_cost(j, stream_);
+        if (e == hipSuccess) e = launch_paths(j, trash_, stream_);
+        if (e == hipSuccess) e = launch_wta(j, stream_);
+        if (e == hipSuccess) e = launch_post(j, stream_);
+        if (e != hipSuccess) return fail(e, "per-pair launch");
+        e = hipEventRecord(done_, stream_);
+        if (e != hipSuccess) return fail(e, "event record");
+        pending_ = true;
+        return LUMEN_OK;
+    }
     build_batch_plan(pl);
     // Tables grow (and move, which forces a new capture) only when a batch outgrows them.
     hipError_t e = grow(jobs_, jobsCap_, npairs);

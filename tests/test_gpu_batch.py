@@ -88,6 +88,22 @@ def test_batch_splits_when_arena_is_small(gpu_available):
         np.testing.assert_array_equal(g, w, err_msg=f"pair {i}")
 
 
+def test_batch_of_one_pair_between_larger_batches(gpu_available):
+    """A batch of one pair takes the per-pair kernels; runs of one pair and of several on one
+    engine, in any order, all equal the per-pair path."""
+    b = ls.BatchSGBM(arena_bytes=512 << 20)
+    singles = mixed_pairs(4, 6)
+    p = ls.StereoParams(maxDisparity=48, blockSize=65, uniquenessRatio=0, speckleWindowSize=0)
+    left, right = ls.synth_pair(120, 40, 2, 40, seed=61)
+    singles.append((p, left, right))          # SAD window beyond the LDS segment
+    many = mixed_pairs(7, 7)
+    want_many = plain(many)
+    for i, pr in enumerate(singles):
+        np.testing.assert_array_equal(b.run([pr])[0], plain([pr])[0], err_msg=f"single {i}")
+        for k, (g, w) in enumerate(zip(b.run(many), want_many)):
+            np.testing.assert_array_equal(g, w, err_msg=f"after single {i}: pair {k}")
+
+
 def test_batch_rejects_invalid_pair(gpu_available):
     pairs = mixed_pairs(2, 5)
     bad = ls.StereoParams(minDisparity=10, maxDisparity=10)
