@@ -308,9 +308,10 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
 def bench_scene(ls, torch, W, H, reps):
     """The stages either side of the matcher on one W x H pair: the rectification remap of both
     images (one launch), disparity -> depth, the colour map, and the fused depth + colour pass.
-    Each launch is bracketed by its own hipEvents on the stream it runs on; the figure is the
-    mean launch time over `reps` launches. Algorithmic bytes per pixel: remap 8 (map) + 1 (source
-    tap, the image is read once) + 1 (write) per image; depth 2 + 4; colour 4 + 3; fused 2 + 3."""
+    Each stage is launched `reps` times back to back between two hipEvents on the stream it runs
+    on (the rectifier's own stream, torch's current stream for the others); the figure is the
+    mean per launch. Algorithmic bytes per pixel: remap 8 (map) + 1 (source tap, the image is read
+    once) + 1 (write) per image; depth 2 + 4; colour 4 + 3; fused 2 + 3."""
     from lumenstereo_amd import scene
 
     dev = torch.device("cuda:0")
@@ -327,12 +328,7 @@ def bench_scene(ls, torch, W, H, reps):
     dl, dr = torch.from_numpy(left).to(dev), torch.from_numpy(right).to(dev)
     ol, orr = torch.empty_like(dl), torch.empty_like(dr)
     r.rectify_device(dl.data_ptr(), dr.data_ptr(), W, H, 1, ol.data_ptr(), orr.data_ptr())
-    rect_ms = 0.0
-    for _ in range(reps):
-        r.rectify_device(dl.data_ptr(), dr.data_ptr(), W, H, 1, ol.data_ptr(), orr.data_ptr(),
-                         sync=False)
-        rect_ms += r.last_time_ms()
-    rect_ms /= reps
+    rect_stream = torch.cuda.ExternalStream(r.stream())
 
     m = ls.StereoSGBM(workload_params(ls, 128, True, 8))
     d_disp = torch.empty((H, W), dtype=torch.int16, device=dev)
@@ -343,30 +339,31 @@ def bench_scene(ls, torch, W, H, reps):
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
     launches = {
-        "depth": lambda: scene.disparityToDepthGPU(d_disp.data_ptr(), dz.data_ptr(), W, H, f, B,
-                                                   0.0, s),
-        "colour_map": lambda: scene.colorizeDepthGPU(dz.data_ptr(), dc.data_ptr(), W, H, lo, hi, s),
-        "depth_colour_fused": lambda: scene.depthColorizeGPU(d_disp.data_ptr(), None,
-                                                             dc.data_ptr(), W, H, f, B, 0.0, lo,
-                                                             hi, s),
+        "rectify_pair": (rect_stream, lambda: r.rectify_device(
+            dl.data_ptr(), dr.data_ptr(), W, H, 1, ol.data_ptr(), orr.data_ptr(), sync=False)),
+        "depth": (stream, lambda: scene.disparityToDepthGPU(d_disp.data_ptr(), dz.data_ptr(), W,
+                                                            H, f, B, 0.0, s)),
+        "colour_map": (stream, lambda: scene.colorizeDepthGPU(dz.data_ptr(), dc.data_ptr(), W, H,
+                                                              lo, hi, s)),
+        "depth_colour_fused": (stream, lambda: scene.depthColorizeGPU(
+            d_disp.data_ptr(), None, dc.data_ptr(), W, H, f, B, 0.0, lo, hi, s)),
     }
     times = {}
-    for name, fn in launches.items():
+    for name, (st, fn) in launches.items():
         fn()
-        evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-               for _ in range(reps)]
-        for a, b in evs:
-            a.record(stream)
-            fn()
-            b.record(stream)
         torch.cuda.synchronize()
-        times[name] = sum(a.elapsed_time(b) for a, b in evs) / reps
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        for _ in range(reps):
+            fn()
+        b.record(st)
+        torch.cuda.synchronize()
+        times[name] = a.elapsed_time(b) / reps
     npx = W * H
     per_px = {"rectify_pair": 20, "depth": 6, "colour_map": 7, "depth_colour_fused": 5}
-    times["rectify_pair"] = rect_ms
     out = {"workload": f"{W}x{H} grey pair: rectify both images (one launch), then the "
                        f"{W}x{H} D=128 Census 8-path disparity -> depth -> colour",
-           "timing": f"mean of {reps} launches, hipEvents around each on its stream"}
+           "timing": f"{reps} launches back to back between hipEvents on their stream, mean"}
     for name in ("rectify_pair", "depth", "colour_map", "depth_colour_fused"):
         ms = times[name]
         gbs = per_px[name] * npx / (ms * 1e-3) / 1e9
