@@ -191,7 +191,10 @@ int lumen_batch_arena_bytes(const lumen_batch* b, size_t* out);
  * lumen_sgbm_compute validates them. */
 int lumen_batch_run(lumen_batch* b, const lumen_pair_desc* pairs, int npairs, int device_io);
 
-/* Times of the last run in ms: out[0] total, out[1] upload, out[2] compute, out[3] download. */
+/* Times of the last run in ms: out[0] total (host wall clock of lumen_batch_run), out[1]
+ * upload, out[2] compute, out[3] download. A host-memory batch of 4 or more pairs runs as 2 or 4
+ * chunks alternating over two streams, so that one chunk's staging and transfers overlap another
+ * chunk's kernels; out[1..3] are then sums over the chunks and may add up to more than out[0]. */
 int lumen_batch_get_timings(const lumen_batch* b, float out[4]);
 
 const char* lumen_batch_last_error(const lumen_batch* b);

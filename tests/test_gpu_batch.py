@@ -104,6 +104,20 @@ def test_batch_of_one_pair_between_larger_batches(gpu_available):
             np.testing.assert_array_equal(g, w, err_msg=f"after single {i}: pair {k}")
 
 
+def test_host_batch_in_pipelined_chunks(gpu_available):
+    """Host-memory batches of 16 or more pairs run as four chunks alternating over two engines
+    (2 to 15 pairs: two chunks); every output equals the per-pair path, on repeated runs."""
+    b = ls.BatchSGBM(arena_bytes=1 << 30)
+    for n, seed in ((20, 8), (5, 9), (17, 10)):
+        pairs = mixed_pairs(n, seed)
+        want = plain(pairs)
+        for rep in range(2):
+            for i, (g, w) in enumerate(zip(b.run(pairs), want)):
+                np.testing.assert_array_equal(g, w, err_msg=f"{n} pairs, run {rep}: pair {i}")
+    t = b.timings()
+    assert t[0] > 0 and t[2] > 0
+
+
 def test_batch_rejects_invalid_pair(gpu_available):
     pairs = mixed_pairs(2, 5)
     bad = ls.StereoParams(minDisparity=10, maxDisparity=10)
