@@ -127,7 +127,9 @@ __device__ __forceinline__ float parabola(float s, uint32_t a0, uint32_t a1, uin
 // consecutive disparities of one pixel as one 16-byte store.
 // ---------------------------------------------------------------------------------------
 constexpr int kCensusTile = 256;  // default pixels per census cost segment (batched form)
-constexpr int kSadTile = 64;      // pixels per SAD cost segment
+// Pixels per SAD cost segment. Wider segments (128, 256) were measured slower at every range
+// tried (64 to 192): fewer blocks overlap less of each other's row staging.
+constexpr int kSadTile = 64;
 
 // Census output is computed 64 pixels (one per lane) at a time into an LDS tile whose rows are
 // Dp + 16 bytes apart, so the 16-byte writes of 16 consecutive lanes start on 16 different
@@ -303,10 +305,11 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     const int cap = j.preFilterCap > 1 ? j.preFilterCap : 1;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int rows = 2 * r + 1, nq = (rows + 3) / 4;
-    const int TL = kSadTile + 2 * r, TR = kSadTile + 2 * r + Dp - 1;
+    constexpr int T = kSadTile;
+    const int TL = T + 2 * r, TR = T + 2 * r + Dp - 1;
     uint32_t* lt = reinterpret_cast<uint32_t*>(shs);  // TL columns x nq dwords
     uint32_t* rt = lt + TL * nq;                          // TR columns x nq dwords
-    uint8_t* ot = reinterpret_cast<uint8_t*>(rt + TR * nq);  // [kSadTile][Dp]
+    uint8_t* ot = reinterpret_cast<uint8_t*>(rt + TR * nq);  // [T][Dp]
     const int lbase = x0 - r, rbase = x0 - r - m - (Dp - 1);
     uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
     uint8_t* rb = reinterpret_cast<uint8_t*>(rt);
@@ -329,10 +332,10 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     // floor(s / full) == umulhi(s, ceil(2^32 / full)) for s <= 255 * full (255 * full^2 < 2^32).
     const uint32_t magic = static_cast<uint32_t>((0x100000000ull + full - 1) / full);
     const int parts = max(1, nt / Dp);
-    const int plen = (kSadTile + parts - 1) / parts;
+    const int plen = (T + parts - 1) / parts;
     for (int t = tid; t < Dp * parts; t += nt) {
         const int d = t % Dp, part = t / Dp;
-        const int i0 = part * plen, i1 = min(kSadTile, i0 + plen);
+        const int i0 = part * plen, i1 = min(T, i0 + plen);
         if (i0 >= i1) continue;
         if (d >= D) {
             for (int i = i0; i < i1; ++i) ot[i * Dp + d] = 255;
@@ -371,6 +374,20 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
         // Window of output pixel i covers tile columns i .. i + 2r.
         if constexpr (R > 0) {
             constexpr int NW = 2 * R + 1;
+            // Outputs i in [ilo, ihi) have every tap inside the image and inside the right
+            // image's reachable columns (x - R >= m + d, x + R <= W - 1, all rows present):
+            // their columns need no check and their divisor is the full window.
+            const int ilo = max(i0, a + R - x0);
+            const int ihi = nrows == rows ? min(i1, W - R - x0) : ilo;
+            uint8_t* od = ot + d;
+            auto col_in = [&](int c) -> uint32_t {
+                const uint32_t* L = lt + c * nq;
+                const uint32_t* Rr = rt + (c + Dp - 1 - d) * nq;
+                uint32_t acc = 0;
+#pragma unroll
+                for (int q = 0; q < (2 * R + 1 + 3) / 4; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
+                return acc;
+            };
             uint32_t ring[NW];
             uint32_t sum = 0;
 #pragma unroll
@@ -380,14 +397,25 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
             }
             ring[NW - 1] = 0;
             for (int i = i0; i < i1; i += NW) {
+                if (i >= ilo && i + NW <= ihi) {
 #pragma unroll
-                for (int u = 0; u < NW; ++u) {
-                    if (i + u < i1) {
-                        const int slot = (u + NW - 1) % NW;  // column i + u + 2R enters here
-                        ring[slot] = col(i + u + 2 * R);
+                    for (int u = 0; u < NW; ++u) {
+                        const int slot = (u + NW - 1) % NW;
+                        ring[slot] = col_in(i + u + 2 * R);
                         sum += ring[slot];
-                        emit(i + u, sum);
-                        sum -= ring[u];                       // column i + u leaves
+                        od[(i + u) * Dp] = static_cast<uint8_t>(__umulhi(sum, magic));
+                        sum -= ring[u];
+                    }
+                } else {
+#pragma unroll
+                    for (int u = 0; u < NW; ++u) {
+                        if (i + u < i1) {
+                            const int slot = (u + NW - 1) % NW;  // column i + u + 2R enters here
+                            ring[slot] = col(i + u + 2 * R);
+                            sum += ring[slot];
+                            emit(i + u, sum);
+                            sum -= ring[u];                       // column i + u leaves
+                        }
                     }
                 }
             }
@@ -403,7 +431,7 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     }
     __syncthreads();
     // The segment's costs are contiguous in the volume: 16-byte stores.
-    const int npix = min(kSadTile, W - x0);
+    const int npix = min(T, W - x0);
     const int n16 = npix * Dp / 16;
     LUMEN_G u32x4* dst = reinterpret_cast<LUMEN_G u32x4*>(gmem(j.cost) + ((size_t)y * W + x0) * Dp);
     const u32x4* src = reinterpret_cast<const u32x4*>(ot);
