@@ -34,6 +34,25 @@ def test_remap_host_form_matches_oracle(gpu_available, shape):
     assert np.array_equal(got, oracle.remap(src, mx, my))
 
 
+def test_remap_smooth_and_scattered_maps(gpu_available):
+    """Smooth rectification-like maps (rotation, scale, shift; partly outside the image at the
+    borders), maps scattered over the whole image, a mix of both within one image, a constant
+    map left of the image and a stretching map all equal the oracle."""
+    H, W = 211, 333
+    src = rng.integers(0, 256, (H, W), dtype=np.uint8)
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    cx, cy = W / 2, H / 2
+    smooth_x = (cx + (xs - cx) * 1.01 - (ys - cy) * 0.02 - 1.3).astype(np.float32)
+    smooth_y = (cy + (ys - cy) * 0.99 + (xs - cx) * 0.03 + 0.7).astype(np.float32)
+    scat_x = rng.uniform(-3, W + 3, (H, W)).astype(np.float32)
+    scat_y = rng.uniform(-3, H + 3, (H, W)).astype(np.float32)
+    mix_x = np.where(ys < H // 2, smooth_x, scat_x).astype(np.float32)
+    mix_y = np.where(ys < H // 2, smooth_y, scat_y).astype(np.float32)
+    for mx, my in ((smooth_x, smooth_y), (scat_x, scat_y), (mix_x, mix_y),
+                   (xs * 0 - 5.0, ys), (xs * 3.0, ys * 0.5)):
+        assert np.array_equal(scene.rectifyImageGPU(src, mx, my), oracle.remap(src, mx, my))
+
+
 def test_remap_identity_and_half_shift(gpu_available):
     """tests/test_rectification.cpp:189-265: identity exact; half-pixel shift within 1 of
     cv::remap (here: equal to the closed form, which the oracle tests pin)."""
