@@ -78,6 +78,8 @@ def test_params_roundtrip_without_gpu():
     (dict(maxDisparity=544), (8, 8),
      "Disparity range (maxDisparity - minDisparity) exceeds GPU limit (512). Reduce the search range."),
     (dict(minDisparity=-4, maxDisparity=60), (8, 8), "minDisparity must be >= 0"),
+    (dict(maxDisparity=16), (4, 4601), "Image width exceeds 4600"),
+    (dict(maxDisparity=16), (3, 1), "Image width must be at least 2 for the SAD prefilter"),
 ])
 def test_validation_messages(params, shape, msg):
     m = ls.StereoSGBM(ls.StereoParams(**params))

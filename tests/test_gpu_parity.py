@@ -31,6 +31,8 @@ CASES = [
     (120, 40, 5, 295, 0, 4, {}),          # D = 290: 20 per lane, padded to 320
     (100, 30, 0, 420, 1, 4, {}),          # D = 420: 28 per lane (7 chunks of 4), padded to 448
     (96, 24, 0, 512, 0, 8, {}),           # D = 512, the largest range: 32 per lane
+    (4600, 5, 0, 320, 1, 4, {}),          # the widest image: right-WTA row state fills LDS
+    (4600, 5, 7, 263, 0, 8, {}),          # widest image, 16 per lane, minDisparity > 0
 ]
 
 
