@@ -330,6 +330,7 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     const int nrows = dyhi - dylo + 1;
     const uint32_t full = static_cast<uint32_t>(rows * rows);
     // floor(s / full) == umulhi(s, ceil(2^32 / full)) for s <= 255 * full (255 * full^2 < 2^32).
+    // A 1x1 window (full = 1) has no 32-bit multiplier: its sum is the cost itself.
     const uint32_t magic = static_cast<uint32_t>((0x100000000ull + full - 1) / full);
     const int parts = max(1, nt / Dp);
     const int plen = (T + parts - 1) / parts;
@@ -364,7 +365,7 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
             const int ncols = dxhi - dxlo + 1;
             uint32_t v;
             if (ncols == rows && nrows == rows)
-                v = __umulhi(sum, magic);
+                v = full == 1 ? sum : __umulhi(sum, magic);
             else if (ncols > 0 && nrows > 0)
                 v = sum / static_cast<uint32_t>(ncols * nrows);
             else

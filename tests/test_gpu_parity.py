@@ -26,6 +26,7 @@ CASES = [
     (200, 40, 0, 130, 1, 8, {}),          # D > W/2, padded to 192
     (64, 200, 0, 16, 0, 8, {}),           # tall image, short diagonals
     (90, 60, 0, 24, 0, 4, {"blockSize": 9}),     # SAD window with the generic radius
+    (90, 60, 4, 36, 0, 8, {"blockSize": 1}),     # 1x1 SAD window (divisor 1)
     (70, 50, 0, 16, 0, 4, {"blockSize": 201}),   # SAD window too large for the LDS segment
     (150, 64, 0, 200, 1, 4, {}),          # 16 disparities per lane
     (120, 40, 5, 295, 0, 4, {}),          # D = 290: 20 per lane, padded to 320
