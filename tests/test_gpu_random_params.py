@@ -1,38 +1,15 @@
-"""Seeded random parameter sweep: 48 small pairs, each with random size, disparity range and
-offset, cost mode, path count, block size (odd and even; the reference does not enforce
-isValid), penalties (including P2 <= P1), prefilter cap, uniqueness, LR tolerance and speckle
-settings, compared with the oracle stage by stage. Catches corner cases that the hand-picked
-parity cases miss."""
+"""Seeded random parameter sweep (cases from random_cases.py): each pair is compared with the
+oracle stage by stage through the per-pair path, and random mixes of the same cases are run
+through the batched and serving forms, which must equal the per-pair path pair by pair. Catches
+corner cases that the hand-picked parity cases miss."""
 import numpy as np
 import pytest
 
 import lumenstereo_amd as ls
 import oracle
+from random_cases import random_case
 
 pytestmark = pytest.mark.gpu
-
-
-def random_case(seed):
-    rng = np.random.default_rng(1000 + seed)
-    W = int(rng.integers(8, 200))
-    H = int(rng.integers(3, 72))
-    m = int(rng.choice([0, 0, 0, rng.integers(1, 24)]))
-    D = int(rng.choice([int(rng.integers(1, 64)), int(rng.integers(64, 200)), 16, 64, 128]))
-    p1 = int(rng.integers(1, 400))
-    p = ls.StereoParams(
-        minDisparity=m, maxDisparity=m + D,
-        blockSize=int(rng.choice([1, 3, 4, 5, 6, 7, 9, 11])),
-        P1=p1, P2=int(rng.choice([p1 + int(rng.integers(1, 2000)), max(1, p1 - 5)])),
-        preFilterCap=int(rng.integers(1, 64)),
-        uniquenessRatio=int(rng.choice([0, 0, 5, 15])),
-        disp12MaxDiff=int(rng.choice([-1, 0, 1, 3])),
-        speckleWindowSize=int(rng.choice([0, 0, 12, 20])), speckleRange=int(rng.choice([1, 4])),
-        numDirections=int(rng.choice([4, 8])),
-        matchingCostMode=ls.MatchingCostMode(int(rng.integers(0, 2))))
-    dmax = max(3.0, min(D + m - 2.0, W / 2.0))
-    left, right = ls.synth_pair(W, H, min(2.0, dmax - 1), dmax, seed=5000 + seed,
-                                lowtex_frac=float(rng.choice([0.0, 0.3])))
-    return p, left, right
 
 
 @pytest.mark.parametrize("seed", range(48))
@@ -47,3 +24,29 @@ def test_random_parameters_match_oracle(gpu_available, seed):
     np.testing.assert_array_equal(m.dump("disp_left"), st["disp_left"], err_msg="left " + ctx)
     np.testing.assert_array_equal(m.dump("disp_right"), st["disp_right"], err_msg="right " + ctx)
     np.testing.assert_array_equal(got, want, err_msg="output " + ctx)
+
+
+@pytest.mark.parametrize("batch", range(4))
+def test_random_batches_match_per_pair(gpu_available, batch):
+    # 12 random cases per batch; a batch mixes every cost mode, path count, window and slice
+    # width the draw gives, so the persistent kernels see them side by side.
+    pairs = [random_case(100 + 12 * batch + i) for i in range(12)]
+    got = ls.BatchSGBM(arena_bytes=256 << 20).run(pairs)
+    for i, ((p, l, r), g) in enumerate(zip(pairs, got)):
+        np.testing.assert_array_equal(g, ls.StereoSGBM(p).compute(l, r),
+                                      err_msg=f"batch {batch} pair {i}: {l.shape[1]}x{l.shape[0]} {p}")
+
+
+def test_random_mix_served_matches_per_pair(gpu_available):
+    pairs = [random_case(200 + i) for i in range(32)]
+    srv = ls.ServeSGBM(pool_bytes=1 << 30, lanes=3, max_batch=8)
+    tickets = [srv.submit(p, l, r) for p, l, r in pairs]
+    outs = []
+    for t in tickets:
+        out, info = srv.wait(t)
+        assert info["status"] == 0
+        outs.append(out)
+    srv.close()
+    for i, ((p, l, r), g) in enumerate(zip(pairs, outs)):
+        np.testing.assert_array_equal(g, ls.StereoSGBM(p).compute(l, r),
+                                      err_msg=f"served pair {i}: {l.shape[1]}x{l.shape[0]} {p}")
