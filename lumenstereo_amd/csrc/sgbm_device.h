@@ -189,18 +189,23 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
     uint8_t* ir = il + 5 * lpitch;                                          // 5 x rpitch bytes
     const bool rowOk = y >= 2 && y < H - 2;
     if (rowOk) {
+        // One staged column (five rows) per thread and iteration, all five loads issued before
+        // the first use; columns outside the image stage zeros (their loads read a clamped,
+        // in-image column).
+        const LUMEN_G uint8_t* l0 = gmem(j.left) + (size_t)(y - 2) * W;
+        const LUMEN_G uint8_t* r0 = gmem(j.right) + (size_t)(y - 2) * W;
+        for (int i = tid; i < lpitch + rpitch; i += nt) {
+            const bool isL = i < lpitch;
+            const int c = isL ? x0 - 2 + i : rs - 2 + (i - lpitch);
+            const bool ok = c >= 0 && c < W;
+            const LUMEN_G uint8_t* src = (isL ? l0 : r0) + min(max(c, 0), W - 1);
+            uint8_t* dst = isL ? il + i : ir + (i - lpitch);
+            const int pitch = isL ? lpitch : rpitch;
+            uint8_t v[5];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) {
-            const LUMEN_G uint8_t* lrow = gmem(j.left) + (size_t)(y - 2 + r) * W;
-            const LUMEN_G uint8_t* rrow = gmem(j.right) + (size_t)(y - 2 + r) * W;
-            for (int i = tid; i < lpitch; i += nt) {
-                const int c = x0 - 2 + i;
-                il[r * lpitch + i] = (c >= 0 && c < W) ? lrow[c] : uint8_t(0);
-            }
-            for (int i = tid; i < rpitch; i += nt) {
-                const int c = rs - 2 + i;
-                ir[r * rpitch + i] = (c >= 0 && c < W) ? rrow[c] : uint8_t(0);
-            }
+            for (int r = 0; r < 5; ++r) v[r] = src[(size_t)r * W];
+#pragma unroll
+            for (int r = 0; r < 5; ++r) dst[r * pitch] = ok ? v[r] : uint8_t(0);
         }
         __syncthreads();
         for (int i = tid; i < T; i += nt) {
@@ -215,6 +220,8 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
     }
     const int chunks = Dp / 16;
     const int lane = tid & 63, wave = tid >> 6, nwaves = nt >> 6;
+    // Copy-out: thread tid keeps 16-byte chunk cc of every per-th pixel of the pass.
+    const int per = nt / chunks, cc = tid % chunks, pp = tid / chunks;
     for (int p0 = 0; p0 < T && x0 + p0 < W; p0 += kCensusPass) {
         const int i = p0 + lane;
         const int x = x0 + i;
@@ -222,51 +229,47 @@ __device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* s
         const uint32_t a = leftOk ? sl[i] : 0u;
         for (int c = wave; c < chunks; c += nwaves) {
             const int d0 = c * 16;
-            const uint32_t* rc = sr + i + Dp - 1 - d0;  // rc[-k] is the right code for d0 + k
+            // rc[-k] is the right code for d0 + k; index i + Dp - 1 - d0 - k stays in [0, nr)
+            // for every lane and k, so it is read even where the result is then discarded.
+            const uint32_t* rc = sr + i + Dp - 1 - d0;
             uint32_t w[4];
-            if (leftOk && d0 + 15 < D && x - m - d0 - 15 >= 2 && x - m - d0 < W - 2) {
-                // Interior: all 16 disparities valid. (h*255+12)/24 == (h*2718+168) >> 8 for
-                // h in [0,24]; two results per packed 16-bit multiply-add, byte 1 of each half
-                // gathered with one byte permute per four results.
+            // (h*255+12)/24 == (h*2718+168) >> 8 for h in [0,24]; two results per packed
+            // 16-bit multiply-add, byte 1 of each half gathered with one byte permute per four.
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t h0 = __builtin_popcount(a ^ rc[-(4 * q + 0)]);
+                const uint32_t h1 = __builtin_popcount(a ^ rc[-(4 * q + 1)]);
+                const uint32_t h2 = __builtin_popcount(a ^ rc[-(4 * q + 2)]);
+                const uint32_t h3 = __builtin_popcount(a ^ rc[-(4 * q + 3)]);
+                const u16x2 k = splat(2718u), b = splat(168u);
+                const u16x2 v01 = as16(h0 | (h1 << 16)) * k + b;
+                const u16x2 v23 = as16(h2 | (h3 << 16)) * k + b;
+                w[q] = __builtin_amdgcn_perm(bits(v23), bits(v01), 0x07050301u);
+            }
+            const int e = x - m - d0;  // right column of disparity d0
+            if (!(leftOk && d0 + 15 < D && e - 15 >= 2 && e < W - 2)) {
+                // Not every disparity of the chunk is valid. The valid k (d = d0 + k) form one
+                // run: d < D, 2 <= x - m - d < W - 2; the others read 255.
+                const int klo = max(0, e - (W - 3)), khi = min(min(15, D - 1 - d0), e - 2);
+                const uint32_t valid =
+                    (leftOk && khi >= klo) ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
+                const uint32_t inv = ~valid;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const uint32_t h0 = __builtin_popcount(a ^ rc[-(4 * q + 0)]);
-                    const uint32_t h1 = __builtin_popcount(a ^ rc[-(4 * q + 1)]);
-                    const uint32_t h2 = __builtin_popcount(a ^ rc[-(4 * q + 2)]);
-                    const uint32_t h3 = __builtin_popcount(a ^ rc[-(4 * q + 3)]);
-                    const u16x2 k = splat(2718u), b = splat(168u);
-                    const u16x2 v01 = as16(h0 | (h1 << 16)) * k + b;
-                    const u16x2 v23 = as16(h2 | (h3 << 16)) * k + b;
-                    w[q] = __builtin_amdgcn_perm(bits(v23), bits(v01), 0x07050301u);
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    uint32_t word = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const int d = d0 + q * 4 + b;
-                        const int rx = x - m - d;
-                        uint32_t v = 255;
-                        if (leftOk && d < D && rx >= 2 && rx < W - 2) {
-                            const uint32_t h = __builtin_popcount(a ^ rc[-(4 * q + b)]);
-                            v = (h * 255u + 12u) / 24u;
-                        }
-                        word |= v << (8 * b);
-                    }
-                    w[q] = word;
+                    // Nibble bit b -> byte b (bit b lands at 8b through the 7b shift), then 0xFF.
+                    const uint32_t n = (inv >> (4 * q)) & 0xFu;
+                    w[q] |= ((n * 0x00204081u) & 0x01010101u) * 0xFFu;
                 }
             }
             *reinterpret_cast<u32x4*>(tile + lane * opitch + d0) = u32x4{w[0], w[1], w[2], w[3]};
         }
         __syncthreads();
         const int npx = min(kCensusPass, W - (x0 + p0));
-        LUMEN_G uint8_t* dst = gmem(j.cost) + ((size_t)y * W + x0 + p0) * Dp;
-        for (int f = tid; f < npx * chunks; f += nt) {
-            const int pi = f / chunks, c = f - pi * chunks;
-            *reinterpret_cast<LUMEN_G u32x4*>(dst + (size_t)f * 16) =
-                *reinterpret_cast<const u32x4*>(tile + pi * opitch + c * 16);
-        }
+        LUMEN_G u32x4* dst =
+            reinterpret_cast<LUMEN_G u32x4*>(gmem(j.cost) + ((size_t)y * W + x0 + p0) * Dp);
+        if (pp < per)
+            for (int pi = pp; pi < npx; pi += per)
+                dst[pi * chunks + cc] = *reinterpret_cast<const u32x4*>(tile + pi * opitch + cc * 16);
         __syncthreads();
     }
 }
@@ -311,19 +314,55 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     uint32_t* rt = lt + TL * nq;                          // TR columns x nq dwords
     uint8_t* ot = reinterpret_cast<uint8_t*>(rt + TR * nq);  // [T][Dp]
     const int lbase = x0 - r, rbase = x0 - r - m - (Dp - 1);
-    uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
-    uint8_t* rb = reinterpret_cast<uint8_t*>(rt);
-    for (int i = tid; i < TL * nq * 4; i += nt) {
-        const int c = i / (nq * 4), k = i - c * (nq * 4);
-        const int gc = lbase + c, yy = y - r + k;
-        lb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
-                    ? prefilter_at(gmem(j.left) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
-    }
-    for (int i = tid; i < TR * nq * 4; i += nt) {
-        const int c = i / (nq * 4), k = i - c * (nq * 4);
-        const int gc = rbase + c, yy = y - r + k;
-        rb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
-                    ? prefilter_at(gmem(j.right) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+    if constexpr (R > 0) {
+        // One column of the tiles per thread and iteration: the 2R+1 rows' neighbour pairs are
+        // loaded before the first use (clamped indices, so every load is inside the image),
+        // prefiltered, and packed four rows per dword. Clamping the neighbours reproduces
+        // prefilter_at's edge rule: column 0 takes row[1] - row[0], column W-1 row[W-1] -
+        // row[W-2]. Taps outside the image stage 0 in both tiles and add nothing.
+        constexpr int NQ = (2 * R + 1 + 3) / 4;
+        for (int c = tid; c < TL + TR; c += nt) {
+            const bool isL = c < TL;
+            const int gc = isL ? lbase + c : rbase + (c - TL);
+            const bool colOk = gc >= 0 && gc < W;
+            const int cl = min(max(gc - 1, 0), W - 1), cr = min(max(gc + 1, 0), W - 1);
+            const LUMEN_G uint8_t* img = gmem(isL ? j.left : j.right);
+            uint32_t a[2 * R + 1], b[2 * R + 1];
+#pragma unroll
+            for (int k = 0; k <= 2 * R; ++k) {
+                const size_t row = (size_t)min(max(y - R + k, 0), H - 1) * W;
+                a[k] = img[row + cr];
+                b[k] = img[row + cl];
+            }
+            uint32_t w[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) w[q] = 0;
+#pragma unroll
+            for (int k = 0; k <= 2 * R; ++k) {
+                const int yy = y - R + k;
+                const int g = min(cap, max(-cap, static_cast<int>(a[k]) - static_cast<int>(b[k])));
+                const uint32_t v = (colOk && yy >= 0 && yy < H) ? static_cast<uint32_t>(g + cap) : 0u;
+                w[k / 4] |= v << (8 * (k % 4));
+            }
+            uint32_t* dst = isL ? lt + c * NQ : rt + (c - TL) * NQ;
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) dst[q] = w[q];
+        }
+    } else {
+        uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
+        uint8_t* rb = reinterpret_cast<uint8_t*>(rt);
+        for (int i = tid; i < TL * nq * 4; i += nt) {
+            const int c = i / (nq * 4), k = i - c * (nq * 4);
+            const int gc = lbase + c, yy = y - r + k;
+            lb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
+                        ? prefilter_at(gmem(j.left) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+        }
+        for (int i = tid; i < TR * nq * 4; i += nt) {
+            const int c = i / (nq * 4), k = i - c * (nq * 4);
+            const int gc = rbase + c, yy = y - r + k;
+            rb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
+                        ? prefilter_at(gmem(j.right) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+        }
     }
     __syncthreads();
     const int dylo = max(-r, -y), dyhi = min(r, H - 1 - y);

@@ -63,19 +63,36 @@ hipError_t launch_cost(const PairJob& j, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------- WTA
-template <int K, bool PAD, int NP>
-__global__ __launch_bounds__(256) void wta_row_kernel(PairJob j) {
+// One block per row; NT threads (NT/16 pixels in flight per step).
+template <int K, bool PAD, int NP, int NT>
+__global__ __launch_bounds__(NT) void wta_row_kernel(PairJob j) {
     extern __shared__ unsigned long long rbest[];
     wta_row<K, PAD, NP>(j, blockIdx.x, rbest);
 }
 
+// Frames with fewer rows than kWtaWideRows (about two per CU) give each row 512 threads, so a
+// row takes half the sequential steps; the arithmetic and the atomics' outcome do not depend
+// on the block size.
+constexpr int kWtaWideRows = 512;
+
 template <int K, int NP>
 static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s) {
     const size_t lds = wta_lds_bytes<K>(j.W);
-    if (j.D == j.Dp)
-        hipLaunchKernelGGL((wta_row_kernel<K, false, NP>), dim3(j.H), dim3(256), lds, s, j);
-    else
-        hipLaunchKernelGGL((wta_row_kernel<K, true, NP>), dim3(j.H), dim3(256), lds, s, j);
+    const bool pad = j.D != j.Dp;
+    if constexpr (K <= 8) {
+        if (j.H < kWtaWideRows) {
+            if (pad)
+                hipLaunchKernelGGL((wta_row_kernel<K, true, NP, 512>), dim3(j.H), dim3(512), lds, s, j);
+            else
+                hipLaunchKernelGGL((wta_row_kernel<K, false, NP, 512>), dim3(j.H), dim3(512), lds, s, j);
+            return hipGetLastError();
+        }
+    }
+    if (pad) {
+        hipLaunchKernelGGL((wta_row_kernel<K, true, NP, 256>), dim3(j.H), dim3(256), lds, s, j);
+    } else {
+        hipLaunchKernelGGL((wta_row_kernel<K, false, NP, 256>), dim3(j.H), dim3(256), lds, s, j);
+    }
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ CASES = [
     (100, 70, 0, 32, 0, 8, {"speckleWindowSize": 20, "speckleRange": 2}),
     (200, 40, 0, 130, 1, 8, {}),          # D > W/2, padded to 192
     (64, 200, 0, 16, 0, 8, {}),           # tall image, short diagonals
+    (40, 520, 0, 16, 1, 8, {}),           # 520 rows: the WTA's 256-thread row blocks
     (90, 60, 0, 24, 0, 4, {"blockSize": 9}),     # SAD window with the generic radius
     (90, 60, 4, 36, 0, 8, {"blockSize": 1}),     # 1x1 SAD window (divisor 1)
     (70, 50, 0, 16, 0, 4, {"blockSize": 201}),   # SAD window too large for the LDS segment
