@@ -232,7 +232,9 @@ int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dou
     if (e != hipSuccess) return h->hip_fail(e, "WTA launch");
     e = hipEventRecord(h->ev[slot][4], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
-    e = launch_post(j, s);
+    // The dump hook reads the output from the handle's own buffer: when the caller's buffer is
+    // another one, the median writes both.
+    e = launch_post(j, s, dout != h->out.as<int16_t>() ? h->out.as<int16_t>() : nullptr);
     if (e != hipSuccess) return h->hip_fail(e, "post-processing launch");
     e = hipEventRecord(h->ev[slot][5], s);
     if (e != hipSuccess) return h->hip_fail(e, "event record");
@@ -439,12 +441,6 @@ int lumen_sgbm_compute_device(lumen_sgbm* h, const uint8_t* d_left, const uint8_
     }
     rc = run_stages(h, d_left, d_right, d_out, slot);
     if (rc) return rc;
-    // The dump hook reads the output from the handle's own buffer.
-    if (d_out != h->out.as<int16_t>()) {
-        hipError_t e = hipMemcpyAsync(h->out.ptr, d_out, h->geo.pixels() * 2,
-                                      hipMemcpyDeviceToDevice, s);
-        if (e != hipSuccess) return h->hip_fail(e, "output copy");
-    }
     if (sync) {
         hipError_t e = hipStreamSynchronize(s);
         if (e != hipSuccess) return h->hip_fail(e, "compute");

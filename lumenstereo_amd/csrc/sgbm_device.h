@@ -1140,8 +1140,10 @@ __device__ __forceinline__ void cswap(int& a, int& b) {
     a = lo;
 }
 
+// `mirror`, when not null, receives a second copy of the output.
 __device__ void median_tile(const LUMEN_G int16_t* __restrict__ in, LUMEN_G int16_t* __restrict__ out, int W,
-                            int H, int x0, int y0, int16_t (*tile)[kMedTW + 2]) {
+                            int H, int x0, int y0, int16_t (*tile)[kMedTW + 2],
+                            LUMEN_G int16_t* __restrict__ mirror = nullptr) {
     for (int i = threadIdx.x; i < (kMedTH + 2) * (kMedTW + 2); i += blockDim.x) {
         const int ty = i / (kMedTW + 2), tx = i % (kMedTW + 2);
         const int gx = x0 + tx - 1, gy = y0 + ty - 1;
@@ -1172,7 +1174,9 @@ __device__ void median_tile(const LUMEN_G int16_t* __restrict__ in, LUMEN_G int1
         int r = v[0];
 #pragma unroll
         for (int q = 1; q < 9; ++q) r = k == q ? v[q] : r;
-        out[(size_t)y * W + x] = n == 0 ? tile[ty + 1][tx + 1] : static_cast<int16_t>(r);
+        const int16_t o = n == 0 ? tile[ty + 1][tx + 1] : static_cast<int16_t>(r);
+        out[(size_t)y * W + x] = o;
+        if (mirror) mirror[(size_t)y * W + x] = o;
     }
 }
 

@@ -68,7 +68,8 @@ struct PairJob {
 hipError_t launch_cost(const PairJob& j, hipStream_t s);
 hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s);
 hipError_t launch_wta(const PairJob& j, hipStream_t s);
-hipError_t launch_post(const PairJob& j, hipStream_t s);
+// mirror (may be null): a second output buffer the median also writes.
+hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror = nullptr);
 hipError_t launch_path_sum(const uint16_t* L, size_t pathStride, int npaths, uint16_t* S,
                            size_t n, hipStream_t s);
 // Number of scanlines of SGM path `path` (0..7) over a W x H image.

@@ -123,13 +123,13 @@ __global__ __launch_bounds__(256) void speckle_kernel(PairJob j) {
     speckle_px(gmem(j.dispLR), gmem(j.spk), j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
 }
 
-__global__ __launch_bounds__(256) void median3x3_kernel(PairJob j) {
+__global__ __launch_bounds__(256) void median3x3_kernel(PairJob j, int16_t* mirror) {
     __shared__ int16_t tile[kMedTH + 2][kMedTW + 2];
     median_tile(gmem(j.doSpeckle ? j.spk : j.dispLR), gmem(j.out), j.W, j.H, blockIdx.x * kMedTW,
-                blockIdx.y * kMedTH, tile);
+                blockIdx.y * kMedTH, tile, mirror ? gmem(mirror) : nullptr);
 }
 
-hipError_t launch_post(const PairJob& j, hipStream_t s) {
+hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror) {
     if (j.doSpeckle) {
         dim3 grid((j.W + 15) / 16, (j.H + 15) / 16);
         hipLaunchKernelGGL(speckle_kernel, grid, dim3(256), 0, s, j);
@@ -137,7 +137,7 @@ hipError_t launch_post(const PairJob& j, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     dim3 grid((j.W + kMedTW - 1) / kMedTW, (j.H + kMedTH - 1) / kMedTH);
-    hipLaunchKernelGGL(median3x3_kernel, grid, dim3(256), 0, s, j);
+    hipLaunchKernelGGL(median3x3_kernel, grid, dim3(256), 0, s, j, mirror);
     return hipGetLastError();
 }
 

@@ -93,3 +93,22 @@ def test_bgr_input(gpu_available):
            rc[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
     np.testing.assert_array_equal(got, oracle.sgbm(p, gl, gr))
     del rng
+
+
+def test_compute_device_buffers(gpu_available):
+    """lumen_sgbm_compute_device: images and output in the caller's device buffers. The output
+    equals the oracle's, and the dump hook sees the same output (the handle keeps its own copy);
+    a second, smaller frame through the same handle does the same."""
+    import torch
+    p = _params(0, 48, 1, 8, {"uniquenessRatio": 10})
+    m = ls.StereoSGBM(p)
+    for (W, H, seed) in [(160, 90, 1), (96, 40, 2)]:
+        left, right = ls.synth_pair(W, H, 2, 40, seed=seed)
+        dl = torch.from_numpy(left).cuda()
+        dr = torch.from_numpy(right).cuda()
+        do = torch.full((H, W), 12345, dtype=torch.int16, device="cuda")
+        torch.cuda.synchronize()
+        m.compute_device(dl.data_ptr(), dr.data_ptr(), W, H, do.data_ptr(), sync=True)
+        got = do.cpu().numpy()
+        np.testing.assert_array_equal(got, oracle.sgbm(p, left, right))
+        np.testing.assert_array_equal(m.dump("output"), got)
