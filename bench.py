@@ -399,7 +399,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the matcher "
-              "runs one pair per GPU and this build has no multi-GPU path)", file=sys.stderr)
+              "runs one pair per GPU and this library has no multi-GPU path)", file=sys.stderr)
         return 2
 
     import torch

@@ -158,8 +158,7 @@ int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int heigh
  * band-limited noise left image, slanted planes plus fronto-parallel occluders as ground
  * truth with disparities in [dmin, dmax], right image = left warped by -d with occlusions
  * filled by fresh noise, gain/bias/noise applied. lowtex_frac of the area gets its texture
- * std reduced to 5. gt may be NULL. Same seed gives the same bytes on every machine that runs
- * this build. */
+ * std reduced to 5. gt may be NULL. The same seed gives the same bytes on every machine. */
 int lumen_synth_pair(int width, int height, float dmin, float dmax, uint64_t seed,
                      float lowtex_frac, uint8_t* left, uint8_t* right, float* gt);
 

@@ -348,10 +348,11 @@ int lumen_sgbm_set_schedule(lumen_sgbm* h, const lumen_schedule* s) {
 
 int lumen_sgbm_get_schedule(const lumen_sgbm* h, int width, int height, lumen_schedule* out) {
     if (!h || !out || width <= 0 || height <= 0) return LUMEN_EINVAL;
-    const int D = h->params.maxDisparity - h->params.minDisparity;
+    const long long D = static_cast<long long>(h->params.maxDisparity) - h->params.minDisparity;
     if (D < 1 || D > LUMEN_MAX_DISPARITY_RANGE) return LUMEN_EINVAL;
     const Schedule s = h->hasSchedule ? h->schedule
-                                      : tuned_schedule(width, padded_range(D), h->params.matchingCostMode,
+                                      : tuned_schedule(width, padded_range(static_cast<int>(D)),
+                                                       h->params.matchingCostMode,
                                                        num_paths(h->params));
     out->agg_prefetch = s.aggPrefetch;
     out->agg_priority = s.aggPriority;
@@ -363,7 +364,7 @@ int lumen_sgbm_get_schedule(const lumen_sgbm* h, int width, int height, lumen_sc
 int lumen_sgbm_footprint(const lumen_stereo_params* params, int width, int height,
                          size_t* bytes_out) {
     if (!params || !bytes_out || width <= 0 || height <= 0) return LUMEN_EINVAL;
-    const int D = params->maxDisparity - params->minDisparity;
+    const long long D = static_cast<long long>(params->maxDisparity) - params->minDisparity;
     if (D < 1 || D > LUMEN_MAX_DISPARITY_RANGE) return LUMEN_EINVAL;
     *bytes_out = pair_footprint(*params, width, height);
     return LUMEN_OK;

@@ -97,10 +97,11 @@ inline int16_t* layout_pair(PairJob& j, char* base) {
 }
 
 // Validation of src/sgbm_gpu.cu:148-166 plus the limits of this implementation; returns ""
-// when the pair is accepted, otherwise the message the reference (or this build) gives.
+// when the pair is accepted, otherwise the message the reference (or this implementation) gives.
 inline std::string validate_pair(const lumen_stereo_params& p, int W, int H, bool haveImages) {
     if (!haveImages || W <= 0 || H <= 0) return "Input images are empty";
-    const int D = p.maxDisparity - p.minDisparity;
+    // 64-bit, so that extreme parameter values cannot overflow the difference.
+    const long long D = static_cast<long long>(p.maxDisparity) - p.minDisparity;
     if (D < 1) return "Invalid disparity range: maxDisparity must be greater than minDisparity";
     if (D > LUMEN_MAX_DISPARITY_RANGE)
         return "Disparity range (maxDisparity - minDisparity) exceeds GPU limit (" +

@@ -8,7 +8,7 @@
 // noise image, then gain U[0.95,1.05], bias U[-5,5] and N(0,2) noise.
 //
 // Randomness comes from splitmix64 seeding a xoshiro256** generator and Box-Muller normals, so
-// one seed gives the same images wherever this build runs.
+// one seed gives the same images on every machine.
 #include "lumen_sgbm.h"
 
 #include <algorithm>

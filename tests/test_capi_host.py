@@ -80,6 +80,11 @@ def test_params_roundtrip_without_gpu():
     (dict(minDisparity=-4, maxDisparity=60), (8, 8), "minDisparity must be >= 0"),
     (dict(maxDisparity=16), (4, 4601), "Image width exceeds 4600"),
     (dict(maxDisparity=16), (3, 1), "Image width must be at least 2 for the SAD prefilter"),
+    # Ranges whose difference does not fit 32 bits are still classified by their true value.
+    (dict(minDisparity=-2**31, maxDisparity=2**31 - 1), (8, 8),
+     "Disparity range (maxDisparity - minDisparity) exceeds GPU limit (512). Reduce the search range."),
+    (dict(minDisparity=10, maxDisparity=-2**31 + 5), (8, 8),
+     "Invalid disparity range: maxDisparity must be greater than minDisparity"),
 ])
 def test_validation_messages(params, shape, msg):
     m = ls.StereoSGBM(ls.StereoParams(**params))
@@ -113,6 +118,17 @@ def test_footprint():
     p = ls.StereoParams(maxDisparity=290, numDirections=4).to_c()   # padded to 320
     assert lib.lumen_sgbm_footprint(ctypes.byref(p), 100, 10, ctypes.byref(out)) == 0
     assert out.value == 1000 * 320 * 9 + 1000 * 14
+
+
+def test_footprint_and_schedule_reject_extreme_ranges():
+    lib = ls.library()
+    out = ctypes.c_size_t()
+    for lo, hi in [(-2**31, 2**31 - 1), (10, -2**31 + 5), (0, 0)]:
+        p = ls.StereoParams(minDisparity=lo, maxDisparity=hi).to_c()
+        assert lib.lumen_sgbm_footprint(ctypes.byref(p), 64, 64, ctypes.byref(out)) == ls.LUMEN_EINVAL
+    m = ls.StereoSGBM(ls.StereoParams(minDisparity=-2**31, maxDisparity=2**31 - 1))
+    sched = ls.CSchedule()
+    assert lib.lumen_sgbm_get_schedule(m._h, 64, 64, ctypes.byref(sched)) == ls.LUMEN_EINVAL
 
 
 def test_synth_deterministic_and_shaped():
