@@ -169,7 +169,12 @@ __global__ __launch_bounds__(256) void batch_wta_kernel(const PairJob* __restric
     if (u >= start[npairs]) return;
     const int q = find_pair(start, npairs, u);
     const PairJob j = jobs[pairIdx[q]];
-    wta_dispatch<KSET, 0>(j, u - start[q], rbest);
+    // A compile-time path count lets a pixel's plane loads all be in flight at once and the
+    // next pixel's be issued before this one is summed, as in the per-pair kernel.
+    if (j.npaths == 8)
+        wta_dispatch<KSET, 8>(j, u - start[q], rbest);
+    else
+        wta_dispatch<KSET, 4>(j, u - start[q], rbest);
 }
 
 // ----------------------------------------------------------------------------------- post
