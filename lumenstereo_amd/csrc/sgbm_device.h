@@ -998,13 +998,13 @@ __device__ __forceinline__ void wta_load(const PairJob& j, size_t pix, int l,
     }
 }
 
-// NP is the path count (4 or 8): a compile-time count puts every plane's load of a pixel in
-// flight before the first add, and while a pixel's path costs fit 32 registers the pixel loop
-// is software-pipelined (the next pixel's loads are issued before the current pixel is
-// processed). NP = 0 takes the count from the job and adds each plane as it arrives: fewer
-// registers, for the batched kernel, which holds every slice width in one kernel.
+// NP is the path count (4 or 8; the batched kernel switches on the pair's count): a
+// compile-time count puts every plane's load of a pixel in flight before the first add, and
+// while a pixel's path costs fit 32 registers the pixel loop is software-pipelined (the next
+// pixel's loads are issued before the current pixel is processed).
 template <int K, bool PAD, int NP>
 __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
+    static_assert(NP == 4 || NP == 8, "the path count is 4 or 8");
     // rbest[xr + xr/Q]: running right-WTA key of right pixel xr. One padding entry every Q
     // entries spreads the 16 lanes of a row (whose candidates are Q right pixels apart) over
     // distinct LDS banks.
@@ -1024,34 +1024,7 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
         padm[q] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
     }
     const size_t rowPix = (size_t)y * W;
-    if constexpr (NP == 0) {
-        for (int x = g; x < W; x += ngroups) {
-            const LUMEN_G uint16_t* src = gmem(j.L) + (rowPix + x) * j.Dp + l * Q;
-            uint32_t acc[1][K / 2];
-#pragma unroll
-            for (int q = 0; q < K / 2; ++q) acc[0][q] = 0u;
-            for (int r = 0; r < j.npaths; ++r) {
-                const LUMEN_G uint16_t* p = src + (size_t)r * j.pathStride;
-#pragma unroll
-                for (int ch = 0; ch < K / Q; ++ch) {
-                    if constexpr (Q == 8) {
-                        const u32x4 v = __builtin_nontemporal_load(
-                            reinterpret_cast<const LUMEN_G u32x4*>(p + 16 * Q * ch));
-                        acc[0][4 * ch + 0] = bits(as16(acc[0][4 * ch + 0]) + as16(v.x));
-                        acc[0][4 * ch + 1] = bits(as16(acc[0][4 * ch + 1]) + as16(v.y));
-                        acc[0][4 * ch + 2] = bits(as16(acc[0][4 * ch + 2]) + as16(v.z));
-                        acc[0][4 * ch + 3] = bits(as16(acc[0][4 * ch + 3]) + as16(v.w));
-                    } else {
-                        const u32x2 v = __builtin_nontemporal_load(
-                            reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch));
-                        acc[0][2 * ch + 0] = bits(as16(acc[0][2 * ch + 0]) + as16(v.x));
-                        acc[0][2 * ch + 1] = bits(as16(acc[0][2 * ch + 1]) + as16(v.y));
-                    }
-                }
-            }
-            wta_pixel<K, PAD, 1>(j, x, l, rowPix, acc, padm, sdl, rbest);
-        }
-    } else if constexpr (NP * K / 2 <= 32) {
+    if constexpr (NP * K / 2 <= 32) {
         uint32_t bufA[NP][K / 2], bufB[NP][K / 2];
         int x = g;
         if (x < W) wta_load<K, NP>(j, rowPix + x, l, bufA);
