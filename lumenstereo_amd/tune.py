@@ -10,9 +10,13 @@ and ordered on the device, never what is computed:
 
 For each tuning key (width bucket, padded disparity range Dp, cost mode, path count) the tuner
 times the candidates on the GPU. It is coordinate descent: first prefetch and priority on the
-aggregation stage time, then the census tile on the cost stage time. It checks that every
-candidate's disparity map is bit-identical to the default schedule's and writes the fastest to
-a table that the library reads at first use (``tuning.tsv`` next to liblumen_sgbm.so).
+aggregation stage time, then the census tile on the cost stage time. The candidates of a step
+are timed round-robin over several rounds and each is scored by its median round, so a slow
+drift of the device's speed (which is as large as the differences being measured, a few per
+cent) falls on every candidate alike instead of favouring whichever was timed last. It checks
+that every candidate's disparity map is bit-identical to the default schedule's and writes the
+fastest to a table that the library reads at first use (``tuning.tsv`` next to
+liblumen_sgbm.so).
 
 Run on the GPU box:  python -m lumenstereo_amd.tune --out tuning.tsv --report tuning.md
 """
@@ -92,7 +96,27 @@ def measure(m: StereoSGBM, d_in, shape, reps: int, sched: dict, stage: int):
     return float(t[stage]), do.cpu().numpy()
 
 
-def tune(keys, reps_small: int = 10, log=print):
+def timed_rounds(m: StereoSGBM, d_in, shape, reps: int, cands: list, stage: int, rounds: int):
+    """Median over `rounds` round-robin rounds of each candidate's mean stage time (ms), and
+    each candidate's output."""
+    times = [[] for _ in cands]
+    outs = [None] * len(cands)
+    for _ in range(rounds):
+        for i, c in enumerate(cands):
+            t, out = measure(m, d_in, shape, reps, c, stage)
+            times[i].append(t)
+            if outs[i] is None:
+                outs[i] = out
+    return [float(np.median(v)) for v in times], outs
+
+
+def pick(times: list, base_index: int) -> int:
+    """The fastest candidate if it is clearly (1 %) faster than the default, else the default."""
+    b = min(range(len(times)), key=times.__getitem__)
+    return b if times[b] < 0.99 * times[base_index] else base_index
+
+
+def tune(keys, reps_small: int = 10, rounds: int = 5, log=print):
     import torch
 
     tuning_clear()  # time against the defaults, not against a table already in place
@@ -107,24 +131,21 @@ def tune(keys, reps_small: int = 10, log=print):
         reps = reps_small if W * H * Dp < 400e6 else 4
         m = StereoSGBM(p)
         base = default_schedule(K)
-        t_def_agg, ref = measure(m, d_in, (H, W), reps, base, 2)
-        t_def_cost, _ = measure(m, d_in, (H, W), reps, base, 1)
-        best, best_t = dict(base), t_def_agg
-        mismatches = 0
-        for pf, prio in agg_candidates(K):
-            cand = dict(base, agg_prefetch=pf, agg_priority=prio)
-            t, out = measure(m, d_in, (H, W), reps, cand, 2)
-            mismatches += int(not np.array_equal(out, ref))
-            if t < best_t * 0.99:  # keep the default unless a candidate is clearly faster
-                best, best_t = cand, t
-        best_cost = t_def_cost
-        if census:
-            for tile in (128, 256, 512):
-                cand = dict(best, census_tile=tile)
-                t, out = measure(m, d_in, (H, W), reps, cand, 1)
-                mismatches += int(not np.array_equal(out, ref))
-                if t < best_cost * 0.99:
-                    best, best_cost = cand, t
+        cands = [dict(base)] + [dict(base, agg_prefetch=pf, agg_priority=prio)
+                                for pf, prio in agg_candidates(K)
+                                if (pf, prio) != (base["agg_prefetch"], base["agg_priority"])]
+        times, outs = timed_rounds(m, d_in, (H, W), reps, cands, 2, rounds)
+        ref = outs[0]
+        mismatches = sum(int(not np.array_equal(o, ref)) for o in outs)
+        b = pick(times, 0)
+        best, t_def_agg, best_t = cands[b], times[0], times[b]
+        tiles = (128, 256, 512) if census else (base["census_tile"],)
+        tcands = [dict(best, census_tile=t) for t in tiles]
+        ctimes, couts = timed_rounds(m, d_in, (H, W), reps, tcands, 1, rounds if census else 1)
+        mismatches += sum(int(not np.array_equal(o, ref)) for o in couts)
+        tb = tiles.index(base["census_tile"])
+        c = pick(ctimes, tb)
+        best, t_def_cost, best_cost = tcands[c], ctimes[tb], ctimes[c]
         if mismatches:
             raise RuntimeError(f"{mismatches} schedules changed the output of key {W} {Dp} "
                                f"{int(census)} {paths}")
@@ -151,9 +172,10 @@ def write_table(entries, path: Path, command: str) -> None:
 
 def write_report(report, path: Path, command: str) -> None:
     lines = ["# Schedule tuning report", "", f"Command: `{command}`", "",
-             "Times are means of hipEvent-timed stages over repeated computes of one pair with "
-             "inputs resident in HBM; every candidate's output was compared with the default "
-             "schedule's and was bit-identical.", "",
+             "Times are hipEvent-timed stages over repeated computes of one pair with inputs "
+             "resident in HBM: the mean over the computes of a round, then the median over "
+             "rounds in which the candidates take turns. Every candidate's output was compared "
+             "with the default schedule's and was bit-identical.", "",
              "| W x H | Dp (D) | cost | paths | agg default (ms) | agg tuned (ms) | cost default (ms) "
              "| cost tuned (ms) | schedule |",
              "|---|---|---|---|---|---|---|---|---|"]
@@ -170,10 +192,14 @@ def main(argv=None) -> int:
     ap.add_argument("--report", default=None)
     ap.add_argument("--quick", action="store_true", help="fewer widths (for a smoke run)")
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--rounds", type=int, default=5, help="round-robin rounds per candidate set")
     args = ap.parse_args(argv)
-    command = "python -m lumenstereo_amd.tune " + " ".join(argv if argv is not None else sys.argv[1:])
+    # Recorded with the files' names only, so the table does not carry the tuning machine's paths.
+    command = (f"python -m lumenstereo_amd.tune --reps {args.reps} --rounds {args.rounds}"
+               + (" --quick" if args.quick else "") + f" --out {Path(args.out).name}"
+               + (f" --report {Path(args.report).name}" if args.report else ""))
     t0 = time.time()
-    entries, report = tune(tuning_keys(args.quick), args.reps,
+    entries, report = tune(tuning_keys(args.quick), args.reps, args.rounds,
                            log=lambda s: print(s, file=sys.stderr, flush=True))
     write_table(entries, Path(args.out), command)
     if args.report:
