@@ -1,6 +1,6 @@
 """Seeded random stereo cases shared by the parameter sweeps (imported by the tests, not collected).
 
-Each case draws image size, disparity range and offset, cost mode, path count, block size (odd
+Each case draws image size (a third of them 200-700 px wide), disparity range and offset, cost mode, path count, block size (odd
 and even: the reference does not enforce isValid), penalties (including P2 <= P1), prefilter
 cap, uniqueness, LR tolerance and speckle settings, and a synthetic pair whose true disparities
 fit the range.
@@ -14,6 +14,11 @@ def random_case(seed):
     rng = np.random.default_rng(1000 + seed)
     W = int(rng.integers(8, 200))
     H = int(rng.integers(3, 72))
+    if seed % 3 == 2:
+        # Wider rows (several cost segments per row, census segments of 128-512 px) with fewer
+        # rows, so the oracle stays quick.
+        W = int(rng.integers(200, 700))
+        H = int(rng.integers(3, 24))
     m = int(rng.choice([0, 0, 0, rng.integers(1, 24)]))
     D = int(rng.choice([int(rng.integers(1, 64)), int(rng.integers(64, 200)), 16, 64, 128]))
     p1 = int(rng.integers(1, 400))

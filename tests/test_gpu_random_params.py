@@ -12,7 +12,7 @@ from random_cases import random_case
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("seed", range(48))
+@pytest.mark.parametrize("seed", range(96))
 def test_random_parameters_match_oracle(gpu_available, seed):
     p, left, right = random_case(seed)
     want, st = oracle.sgbm(p, left, right, dumps=True)
