@@ -8,6 +8,10 @@
  * Disparity maps are int16 in 1/16 px (the matcher's output); depth maps are float metres,
  * 0 where invalid; colour maps are 3 bytes per pixel in R, G, B order (the reference's GPU
  * kernel writes its R table first, src/depth_map.cu:134-137).
+ *
+ * Threading: every function may be called from several threads. The host-buffer forms share
+ * one locked device staging area with lumen_remap; the device forms only enqueue on the
+ * caller's stream.
  */
 #pragma once
 #include <stddef.h>

@@ -7,6 +7,10 @@
  * LUMEN_EDEVICE, LUMEN_ENOMEM), no exception crosses the boundary. Errors raised without a
  * handle are reported by lumen_last_error().
  *
+ * Threading: a rectifier handle must not be used by two threads at once. lumen_remap (the
+ * host-buffer form) may be called from several threads: the device staging it shares with the
+ * depth and colour host-buffer calls is locked.
+ *
  * Arithmetic follows remapKernelSimple (src/rectification.cu:201-256 of the reference): the
  * floor of the map coordinate selects four taps, taps outside the image count as 0, the
  * weights (1-dx)(1-dy), (1-dx)dy, dx(1-dy), dx*dy are applied in that order in IEEE single

@@ -7,6 +7,11 @@
  * The text of the last error of a handle is returned by lumen_sgbm_last_error(); errors that
  * occur before a handle exists are reported by lumen_last_error().
  *
+ * Threading: a lumen_sgbm handle, like the reference's StereoSGBM (one stream, member buffers),
+ * must not be used by two threads at once; separate handles are independent. The same holds
+ * for a lumen_batch handle. A lumen_server may be submitted to and waited on from any number of
+ * threads (it locks internally). The tuning table is process-wide and locked.
+ *
  * Citations are into the reference (Pfactorial01/LumenStereo); each entry point names the
  * interface it replaces.
  */
