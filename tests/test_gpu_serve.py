@@ -116,3 +116,38 @@ def test_device_io(gpu_available):
     srv.close()
     for (p, l, r), (_, _, do) in zip(pairs, keep):
         np.testing.assert_array_equal(do.cpu().numpy(), ls.StereoSGBM(p).compute(l, r))
+
+
+def test_submit_and_wait_from_several_threads(gpu_available):
+    """Four threads submit and wait at once on one server (ctypes releases the GIL during the
+    calls); every result equals the per-pair path's."""
+    import threading
+
+    pairs = _mix(32, 11)
+    want = [ls.StereoSGBM(p).compute(l, r) for p, l, r in pairs]
+    srv = ls.ServeSGBM(pool_bytes=1 << 30, lanes=3, max_batch=8)
+    got = [None] * len(pairs)
+    errors = []
+
+    def worker(k):
+        try:
+            mine = list(range(k, len(pairs), 4))
+            tickets = [(i, srv.submit(*pairs[i])) for i in mine]
+            for i, t in tickets:
+                out, info = srv.wait(t)
+                assert info["status"] == 0
+                got[i] = out
+        except Exception as e:  # reported on the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=60)
+    st = srv.stats()
+    srv.close()
+    assert not errors, errors
+    assert st["completed"] == len(pairs) and st["failed"] == 0
+    for i, (g, w) in enumerate(zip(got, want)):
+        np.testing.assert_array_equal(g, w, err_msg=f"pair {i}")
