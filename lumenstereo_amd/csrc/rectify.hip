@@ -73,8 +73,32 @@ __device__ __forceinline__ uint8_t sample(const uint8_t* __restrict__ src, int W
     return static_cast<uint8_t>(static_cast<unsigned>(val + 0.5f));
 }
 
-// Grey images, four pixels per thread. VEC: maps 16-byte and output 4-byte aligned (checked
-// on the host); the last partial group, and unaligned buffers, go pixel by pixel.
+// Grey sample for a 4-byte aligned source of nbytes = W*H bytes. When all four taps are in the
+// image, each row's pair of taps is read as the two dwords at the pair's 4-byte aligned address
+// (the pair sits at byte offset 0-3 of those 8 bytes) and extracted with one byte-align: two
+// loads per row instead of two byte gathers. Same bytes, same arithmetic as sample(); taps at
+// the image's border, and a pair whose 8 bytes would end past the image, take sample().
+__device__ __forceinline__ uint8_t sample_grey(const uint8_t* __restrict__ src, int W, int H,
+                                               long long nbytes, const Taps& t) {
+    if (t.ok && t.x0 >= 0 && t.x0 + 1 < W && t.y0 >= 0 && t.y0 + 1 < H) {
+        const long long a0 = (long long)t.y0 * W + t.x0, a1 = a0 + W;
+        const long long b0 = a0 & ~3LL, b1 = a1 & ~3LL;
+        if (b1 + 8 <= nbytes) {
+            const uint32_t* p0 = reinterpret_cast<const uint32_t*>(src + b0);
+            const uint32_t* p1 = reinterpret_cast<const uint32_t*>(src + b1);
+            const uint32_t r0 = __builtin_amdgcn_alignbyte(p0[1], p0[0], static_cast<uint32_t>(a0 - b0));
+            const uint32_t r1 = __builtin_amdgcn_alignbyte(p1[1], p1[0], static_cast<uint32_t>(a1 - b1));
+            const float v00 = static_cast<float>(r0 & 0xFFu), v10 = static_cast<float>((r0 >> 8) & 0xFFu);
+            const float v01 = static_cast<float>(r1 & 0xFFu), v11 = static_cast<float>((r1 >> 8) & 0xFFu);
+            const float val = t.w00 * v00 + t.w01 * v01 + t.w10 * v10 + t.w11 * v11;
+            return static_cast<uint8_t>(static_cast<unsigned>(val + 0.5f));
+        }
+    }
+    return sample(src, W, H, 1, t, 0);
+}
+
+// Grey images, four pixels per thread. VEC: maps 16-byte, output and source 4-byte aligned
+// (checked on the host); the last partial group, and unaligned buffers, go pixel by pixel.
 template <bool VEC>
 __global__ void __launch_bounds__(kBlock) remap_grey_kernel(RemapPair p, int W, int H, long long n) {
     const RemapImage im = p.img[blockIdx.y];
@@ -83,10 +107,10 @@ __global__ void __launch_bounds__(kBlock) remap_grey_kernel(RemapPair p, int W, 
     if (VEC && i + kPixPerThread <= n) {
         const float4 mx = *reinterpret_cast<const float4*>(im.mapx + i);
         const float4 my = *reinterpret_cast<const float4*>(im.mapy + i);
-        const uint32_t b0 = sample(im.src, W, H, 1, taps_of(mx.x, my.x), 0);
-        const uint32_t b1 = sample(im.src, W, H, 1, taps_of(mx.y, my.y), 0);
-        const uint32_t b2 = sample(im.src, W, H, 1, taps_of(mx.z, my.z), 0);
-        const uint32_t b3 = sample(im.src, W, H, 1, taps_of(mx.w, my.w), 0);
+        const uint32_t b0 = sample_grey(im.src, W, H, n, taps_of(mx.x, my.x));
+        const uint32_t b1 = sample_grey(im.src, W, H, n, taps_of(mx.y, my.y));
+        const uint32_t b2 = sample_grey(im.src, W, H, n, taps_of(mx.z, my.z));
+        const uint32_t b3 = sample_grey(im.src, W, H, n, taps_of(mx.w, my.w));
         *reinterpret_cast<uint32_t*>(im.dst + i) = b0 | (b1 << 8) | (b2 << 16) | (b3 << 24);
         return;
     }
@@ -112,7 +136,7 @@ hipError_t launch_remap(const RemapPair& p, int nimg, int W, int H, int C, hipSt
         bool vec = true;
         for (int k = 0; k < nimg; ++k)
             vec = vec && aligned(p.img[k].mapx, 16) && aligned(p.img[k].mapy, 16) &&
-                  aligned(p.img[k].dst, 4);
+                  aligned(p.img[k].dst, 4) && aligned(p.img[k].src, 4);
         const long long per = (long long)kBlock * kPixPerThread;
         const dim3 grid(static_cast<unsigned>((n + per - 1) / per), nimg);
         if (vec)
