@@ -16,11 +16,36 @@ constexpr size_t kChunk = 1 << 20;           // bytes per unit of work
 constexpr size_t kParallelMin = 4u << 20;    // below this, the caller copies alone
 constexpr unsigned kMaxWorkers = 7;          // plus the caller
 
+// A unit of work: `bytes` bytes copied, or, when rows > 0, `rows` rows of a BGR -> gray
+// conversion (W pixels a row, source rows srcStride bytes apart, destination rows packed).
 struct Chunk {
     char* dst;
     const char* src;
     size_t bytes;
+    int rows = 0;
+    int W = 0;
+    size_t srcStride = 0;
 };
+
+void bgr_rows(const Chunk& c) {
+    const unsigned char* s0 = reinterpret_cast<const unsigned char*>(c.src);
+    unsigned char* d0 = reinterpret_cast<unsigned char*>(c.dst);
+    for (int y = 0; y < c.rows; ++y) {
+        const unsigned char* s = s0 + (size_t)y * c.srcStride;
+        unsigned char* d = d0 + (size_t)y * c.W;
+        for (int x = 0; x < c.W; ++x) {
+            const int b = s[3 * x], g = s[3 * x + 1], r = s[3 * x + 2];
+            d[x] = static_cast<unsigned char>((b * 1868 + g * 9617 + r * 4899 + (1 << 13)) >> 14);
+        }
+    }
+}
+
+void run_chunk(const Chunk& c) {
+    if (c.rows > 0)
+        bgr_rows(c);
+    else
+        std::memcpy(c.dst, c.src, c.bytes);
+}
 
 class Pool {
 public:
@@ -66,7 +91,7 @@ private:
         for (;;) {
             const size_t i = next_.fetch_add(1);
             if (i >= chunks.size()) return;
-            std::memcpy(chunks[i].dst, chunks[i].src, chunks[i].bytes);
+            run_chunk(chunks[i]);
         }
     }
 
@@ -100,8 +125,13 @@ private:
 void split(std::vector<Chunk>& out, void* dst, const void* src, size_t bytes) {
     char* d = static_cast<char*>(dst);
     const char* s = static_cast<const char*>(src);
-    for (size_t off = 0; off < bytes; off += kChunk)
-        out.push_back({d + off, s + off, std::min(kChunk, bytes - off)});
+    for (size_t off = 0; off < bytes; off += kChunk) {
+        Chunk c{};
+        c.dst = d + off;
+        c.src = s + off;
+        c.bytes = std::min(kChunk, bytes - off);
+        out.push_back(c);
+    }
 }
 
 }  // namespace
@@ -133,6 +163,28 @@ void parallel_copy_rows(void* dst, size_t dstStride, const void* src, size_t src
         jobs.push_back({static_cast<char*>(dst) + (size_t)y * dstStride,
                         static_cast<const char*>(src) + (size_t)y * srcStride, rowBytes});
     parallel_copy(jobs);
+}
+
+void parallel_bgr_to_gray(unsigned char* dst, const unsigned char* src, size_t srcStride, int W,
+                          int H) {
+    if (W <= 0 || H <= 0) return;
+    // Bands of about one chunk of source bytes.
+    const int band = std::max(1, static_cast<int>(kChunk / (3 * static_cast<size_t>(W))));
+    std::vector<Chunk> chunks;
+    for (int y = 0; y < H; y += band) {
+        Chunk c{};
+        c.dst = reinterpret_cast<char*>(dst + (size_t)y * W);
+        c.src = reinterpret_cast<const char*>(src + (size_t)y * srcStride);
+        c.rows = std::min(band, H - y);
+        c.W = W;
+        c.srcStride = srcStride;
+        chunks.push_back(c);
+    }
+    if ((size_t)W * H * 3 < kParallelMin) {
+        for (const Chunk& c : chunks) bgr_rows(c);
+        return;
+    }
+    Pool::get().run(chunks);
 }
 
 }  // namespace lumen

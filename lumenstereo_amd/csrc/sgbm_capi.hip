@@ -248,19 +248,6 @@ int begin_slot(lumen_sgbm* h) {
     return slot;
 }
 
-// OpenCV's fixed-point BGR->gray (R 0.299, G 0.587, B 0.114 in 14-bit fixed point, rounded),
-// the conversion src/sgbm_gpu.cu:168-175 applies to 3-channel input.
-void bgr_to_gray(const uint8_t* src, size_t stride, int W, int H, uint8_t* dst) {
-    for (int y = 0; y < H; ++y) {
-        const uint8_t* s = src + (size_t)y * stride;
-        for (int x = 0; x < W; ++x) {
-            const int b = s[3 * x], gch = s[3 * x + 1], r = s[3 * x + 2];
-            dst[(size_t)y * W + x] =
-                static_cast<uint8_t>((b * 1868 + gch * 9617 + r * 4899 + (1 << 13)) >> 14);
-        }
-    }
-}
-
 int check_params(const lumen_stereo_params* p) {
     if (!p) {
         g_last_error = "params is NULL";
@@ -398,8 +385,9 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
     if (e != hipSuccess) return h->hip_fail(e, "pinned output staging");
     uint8_t* pin = h->stageIn.as<uint8_t>();
     if (ch == 3) {
-        bgr_to_gray(left, left_stride, W, H, pin);
-        bgr_to_gray(right, right_stride, W, H, pin + n);
+        // The conversion src/sgbm_gpu.cu:168-175 applies to 3-channel input (OpenCV's BGR2GRAY).
+        parallel_bgr_to_gray(pin, left, left_stride, W, H);
+        parallel_bgr_to_gray(pin + n, right, right_stride, W, H);
     } else {
         parallel_copy_rows(pin, W, left, left_stride, W, H);
         parallel_copy_rows(pin + n, W, right, right_stride, W, H);

@@ -258,6 +258,30 @@ static void parallel_copies() {
     EXPECT(ok);
 }
 
+// BGR -> gray in bands on the pool, against the formula pixel by pixel: below the parallel
+// threshold, above it, with a padded source stride, twice in a row.
+static void parallel_bgr() {
+    std::mt19937 rng(5);
+    const int sizes[][3] = {{7, 5, 21}, {640, 480, 1920}, {1921, 1081, 5800}, {1000, 3, 3000}};
+    for (int rep = 0; rep < 2; ++rep)
+        for (const auto& sz : sizes) {
+            const int W = sz[0], H = sz[1];
+            const size_t stride = static_cast<size_t>(sz[2]);
+            std::vector<uint8_t> src(stride * H);
+            for (auto& v : src) v = static_cast<uint8_t>(rng());
+            std::vector<uint8_t> dst((size_t)W * H, 0);
+            parallel_bgr_to_gray(dst.data(), src.data(), stride, W, H);
+            bool ok = true;
+            for (int y = 0; y < H && ok; ++y)
+                for (int x = 0; x < W; ++x) {
+                    const uint8_t* p = src.data() + (size_t)y * stride + 3 * x;
+                    const int want = (p[0] * 1868 + p[1] * 9617 + p[2] * 4899 + 8192) >> 14;
+                    if (dst[(size_t)y * W + x] != want) ok = false;
+                }
+            EXPECT(ok);
+        }
+}
+
 static void batch_picking() {
     // Oldest first; only requests within 4x of the first's work; total within the budget.
     std::vector<double> w = {10, 100, 12, 3, 30, 41, 9, 11};
@@ -286,6 +310,7 @@ int main() {
         {"admission_simulation", admission_simulation},
         {"percentiles", percentiles},
         {"parallel_copies", parallel_copies},
+        {"parallel_bgr", parallel_bgr},
         {"batch_picking", batch_picking},
     };
     for (auto& c : cases) {

@@ -112,3 +112,18 @@ def test_compute_device_buffers(gpu_available):
         got = do.cpu().numpy()
         np.testing.assert_array_equal(got, oracle.sgbm(p, left, right))
         np.testing.assert_array_equal(m.dump("output"), got)
+
+
+def test_bgr_input_large(gpu_available):
+    """A 1080p BGR pair (converted in parallel bands on the host) gives exactly the output of
+    its gray conversion."""
+    left, right = ls.synth_pair(1920, 1080, 4, 60, seed=21)
+    rng = np.random.default_rng(4)
+    lc = np.stack([left, rng.integers(0, 256, left.shape, dtype=np.uint8), left[:, ::-1]], axis=2).copy()
+    rc = np.stack([right, rng.integers(0, 256, right.shape, dtype=np.uint8), right[:, ::-1]], axis=2).copy()
+    gl = ((lc[..., 0].astype(np.int32) * 1868 + lc[..., 1].astype(np.int32) * 9617 +
+           lc[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
+    gr = ((rc[..., 0].astype(np.int32) * 1868 + rc[..., 1].astype(np.int32) * 9617 +
+           rc[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
+    m = ls.StereoSGBM(_params(0, 64, 1, 4, {}))
+    np.testing.assert_array_equal(m.compute(lc, rc), m.compute(gl, gr))

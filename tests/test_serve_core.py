@@ -1,6 +1,6 @@
 """Runs the host-only test program of the serving loop's device pool and admission logic
 (tests/cpp/test_serve_core.cpp, an injected fake allocator in place of hipMalloc), and of the
-staging copy pool."""
+staging copy pool (copies and BGR -> gray)."""
 import subprocess
 from pathlib import Path
 
@@ -15,4 +15,4 @@ def test_serve_core_host_logic():
     r = subprocess.run([str(BIN)], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert r.stdout.count("ok ") == 8, r.stdout
+    assert r.stdout.count("ok ") == 9, r.stdout
