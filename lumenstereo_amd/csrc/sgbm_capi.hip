@@ -388,6 +388,9 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
         // The conversion src/sgbm_gpu.cu:168-175 applies to 3-channel input (OpenCV's BGR2GRAY).
         parallel_bgr_to_gray(pin, left, left_stride, W, H);
         parallel_bgr_to_gray(pin + n, right, right_stride, W, H);
+    } else if (left_stride == (size_t)W && right_stride == (size_t)W) {
+        // Packed images: one parallel copy of both, so a 1080p pair (4 MB) uses the pool.
+        parallel_copy({{pin, left, n}, {pin + n, right, n}});
     } else {
         parallel_copy_rows(pin, W, left, left_stride, W, H);
         parallel_copy_rows(pin + n, W, right, right_stride, W, H);
