@@ -118,6 +118,52 @@ __global__ void __launch_bounds__(kBlock) remap_grey_kernel(RemapPair p, int W, 
         im.dst[i + k] = sample(im.src, W, H, 1, taps_of(im.mapx[i + k], im.mapy[i + k]), 0);
 }
 
+// Interleaved C-channel images (C = 2..4, source 4-byte aligned): one pixel per thread. When
+// all four taps are in the image, a row's two taps are the 2C bytes at x0*C, read as the three
+// dwords at their 4-byte aligned address (the run starts at byte 0-3 and ends by byte 11) and
+// split with two byte-aligns: six loads a pixel instead of 4C byte gathers. Same bytes and
+// arithmetic as sample(); border taps, and a run whose 12 bytes would end past the image, take
+// sample().
+template <int C>
+__global__ void __launch_bounds__(kBlock) remap_multi_fast_kernel(RemapPair p, int W, int H,
+                                                                  long long n) {
+    static_assert(C >= 2 && C <= 4, "two taps of C channels fit 8 bytes");
+    const RemapImage im = p.img[blockIdx.y];
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const Taps t = taps_of(im.mapx[i], im.mapy[i]);
+    const long long nbytes = n * C;
+    if (t.ok && t.x0 >= 0 && t.x0 + 1 < W && t.y0 >= 0 && t.y0 + 1 < H) {
+        const long long a0 = ((long long)t.y0 * W + t.x0) * C, a1 = a0 + (long long)W * C;
+        const long long b0 = a0 & ~3LL, b1 = a1 & ~3LL;
+        if (b1 + 12 <= nbytes) {
+            // run(row): the 8 bytes starting at the row's first tap, as lo (bytes 0-3), hi (4-7).
+            auto run = [&](long long b, long long a, uint32_t& lo, uint32_t& hi) {
+                const uint32_t* q = reinterpret_cast<const uint32_t*>(im.src + b);
+                const uint32_t sh = static_cast<uint32_t>(a - b);
+                const uint32_t w0 = q[0], w1 = q[1], w2 = q[2];
+                lo = __builtin_amdgcn_alignbyte(w1, w0, sh);
+                hi = __builtin_amdgcn_alignbyte(w2, w1, sh);
+            };
+            uint32_t lo0, hi0, lo1, hi1;
+            run(b0, a0, lo0, hi0);
+            run(b1, a1, lo1, hi1);
+            auto byte = [](uint32_t lo, uint32_t hi, int k) -> float {
+                return static_cast<float>(((k < 4 ? lo : hi) >> (8 * (k & 3))) & 0xFFu);
+            };
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const float v00 = byte(lo0, hi0, c), v10 = byte(lo0, hi0, C + c);
+                const float v01 = byte(lo1, hi1, c), v11 = byte(lo1, hi1, C + c);
+                const float val = t.w00 * v00 + t.w01 * v01 + t.w10 * v10 + t.w11 * v11;
+                im.dst[i * C + c] = static_cast<uint8_t>(static_cast<unsigned>(val + 0.5f));
+            }
+            return;
+        }
+    }
+    for (int c = 0; c < C; ++c) im.dst[i * C + c] = sample(im.src, W, H, C, t, c);
+}
+
 // Interleaved multi-channel images: one pixel per thread, every channel from the same taps.
 __global__ void __launch_bounds__(kBlock) remap_multi_kernel(RemapPair p, int W, int H, int C,
                                                              long long n) {
@@ -144,8 +190,17 @@ hipError_t launch_remap(const RemapPair& p, int nimg, int W, int H, int C, hipSt
         else
             remap_grey_kernel<false><<<grid, kBlock, 0, s>>>(p, W, H, n);
     } else {
+        bool src4 = true;
+        for (int k = 0; k < nimg; ++k) src4 = src4 && aligned(p.img[k].src, 4);
         const dim3 grid(static_cast<unsigned>((n + kBlock - 1) / kBlock), nimg);
-        remap_multi_kernel<<<grid, kBlock, 0, s>>>(p, W, H, C, n);
+        if (src4 && C == 3)
+            remap_multi_fast_kernel<3><<<grid, kBlock, 0, s>>>(p, W, H, n);
+        else if (src4 && C == 4)
+            remap_multi_fast_kernel<4><<<grid, kBlock, 0, s>>>(p, W, H, n);
+        else if (src4 && C == 2)
+            remap_multi_fast_kernel<2><<<grid, kBlock, 0, s>>>(p, W, H, n);
+        else
+            remap_multi_kernel<<<grid, kBlock, 0, s>>>(p, W, H, C, n);
     }
     return hipGetLastError();
 }

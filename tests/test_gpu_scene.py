@@ -25,7 +25,7 @@ def _maps(H, W, lo=-2.0, spread=3.0):
     return mx, my
 
 
-@pytest.mark.parametrize("shape", [(48, 64, 1), (37, 53, 1), (33, 41, 3), (20, 30, 4), (1, 7, 1)])
+@pytest.mark.parametrize("shape", [(48, 64, 1), (37, 53, 1), (33, 41, 3), (20, 30, 4), (29, 35, 2), (1, 7, 1)])
 def test_remap_host_form_matches_oracle(gpu_available, shape):
     H, W, C = shape
     src = rng.integers(0, 256, (H, W) if C == 1 else (H, W, C), dtype=np.uint8)
@@ -51,6 +51,23 @@ def test_remap_smooth_and_scattered_maps(gpu_available):
     for mx, my in ((smooth_x, smooth_y), (scat_x, scat_y), (mix_x, mix_y),
                    (xs * 0 - 5.0, ys), (xs * 3.0, ys * 0.5)):
         assert np.array_equal(scene.rectifyImageGPU(src, mx, my), oracle.remap(src, mx, my))
+
+
+@pytest.mark.parametrize("C", [2, 3, 4])
+def test_remap_colour_smooth_and_scattered_maps(gpu_available, C):
+    """Interleaved images: a smooth rectification-like map, maps scattered over the image and
+    beyond it, and the identity, each equal to the oracle channel by channel."""
+    H, W = 157, 243
+    src = rng.integers(0, 256, (H, W, C), dtype=np.uint8)
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    cx, cy = W / 2, H / 2
+    smooth_x = (cx + (xs - cx) * 1.02 - (ys - cy) * 0.01 - 0.6).astype(np.float32)
+    smooth_y = (cy + (ys - cy) * 0.98 + (xs - cx) * 0.02 + 1.1).astype(np.float32)
+    scat_x = rng.uniform(-3, W + 3, (H, W)).astype(np.float32)
+    scat_y = rng.uniform(-3, H + 3, (H, W)).astype(np.float32)
+    for mx, my in ((smooth_x, smooth_y), (scat_x, scat_y)):
+        assert np.array_equal(scene.rectifyImageGPU(src, mx, my), oracle.remap(src, mx, my))
+    assert np.array_equal(scene.rectifyImageGPU(src, xs, ys), src)
 
 
 def test_remap_identity_and_half_shift(gpu_available):
