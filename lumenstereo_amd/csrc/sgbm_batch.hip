@@ -184,6 +184,7 @@ __global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __res
     const int* start = t.spkStart;
     const int* pairIdx = t.spkPairs;
     const int total = start[npairs];
+    __shared__ int16_t tile[20 * 20];
     PairCursor c;
     for (int u = blockIdx.x; u < total; u += gridDim.x) {
         if (c.seek(start, npairs, u)) c.job = jobs[pairIdx[c.idx]];
@@ -191,9 +192,8 @@ __global__ __launch_bounds__(256) void batch_speckle_kernel(const PairJob* __res
         const int tiles = (j.W + 15) / 16;
         const int local = u - c.beg;
         const int ty = local / tiles, tx = local - ty * tiles;
-        const int x = tx * 16 + (threadIdx.x & 15), y = ty * 16 + (threadIdx.x >> 4);
-        if (x < j.W && y < j.H)
-            speckle_px(gmem(j.dispLR), gmem(j.spk), j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
+        speckle_tile<16, 16>(gmem(j.dispLR), gmem(j.spk), j.W, j.H, tx * 16, ty * 16, j.speckleMin,
+                             j.speckleDiff, tile);
     }
 }
 

@@ -1076,26 +1076,47 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
 
 // ---------------------------------------------------------------------------------------
 // Speckle filter (src/sgbm_kernels.cu:979-1013) with snapshot semantics: reads `in`, writes
-// `out`, so the result does not depend on thread timing. One pixel.
+// `out`, so the result does not depend on thread timing. A pixel with a value > 0 keeps it when
+// at least minSimilar pixels of its 5x5 window (itself included, out-of-image ones skipped)
+// are > 0 and within maxDiff of it, and becomes -1 otherwise.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void speckle_px(const LUMEN_G int16_t* __restrict__ in,
-                                           LUMEN_G int16_t* __restrict__ out,
-                                           int W, int H, int x, int y, int minSimilar,
-                                           int maxDiff) {
-    const int16_t c = in[(size_t)y * W + x];
-    int16_t res = c;
-    if (c > 0) {
-        int n = 0;
-        for (int dy = -2; dy <= 2; ++dy)
-            for (int dx = -2; dx <= 2; ++dx) {
-                const int nx = x + dx, ny = y + dy;
-                if (nx < 0 || nx >= W || ny < 0 || ny >= H) continue;
-                const int16_t v = in[(size_t)ny * W + nx];
-                if (v > 0 && abs(int(c) - int(v)) <= maxDiff) ++n;
-            }
-        if (n < minSimilar) res = -1;
+// One TW x TH tile, block-wide: the tile and a 2-pixel halo of `in` are staged in LDS (`tile`,
+// (TH+4) x (TW+4) entries) with one load per entry, then every pixel counts its 5x5 window
+// there. Out-of-image neighbours are staged as 0, which the count skips (only values > 0
+// count). Every thread of the block must call it with the same tile; it ends with a barrier,
+// so `tile` may be reused at once.
+template <int TW, int TH>
+__device__ void speckle_tile(const LUMEN_G int16_t* __restrict__ in, LUMEN_G int16_t* __restrict__ out,
+                             int W, int H, int x0, int y0, int minSimilar, int maxDiff,
+                             int16_t* tile) {
+    constexpr int PW = TW + 4, PH = TH + 4;
+    for (int i = threadIdx.x; i < PW * PH; i += blockDim.x) {
+        const int ty = i / PW, tx = i - ty * PW;
+        const int gx = x0 + tx - 2, gy = y0 + ty - 2;
+        tile[i] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? in[(size_t)gy * W + gx] : int16_t(0);
     }
-    out[(size_t)y * W + x] = res;
+    __syncthreads();
+    for (int i = threadIdx.x; i < TW * TH; i += blockDim.x) {
+        const int ty = i / TW, tx = i - ty * TW;
+        const int x = x0 + tx, y = y0 + ty;
+        if (x >= W || y >= H) continue;
+        const int16_t* w = tile + ty * PW + tx;  // top-left of the 5x5 window
+        const int16_t c = w[2 * PW + 2];
+        int16_t res = c;
+        if (c > 0) {
+            int n = 0;
+#pragma unroll
+            for (int dy = 0; dy < 5; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 5; ++dx) {
+                    const int16_t v = w[dy * PW + dx];
+                    n += (v > 0 && abs(int(c) - int(v)) <= maxDiff) ? 1 : 0;
+                }
+            if (n < minSimilar) res = -1;
+        }
+        out[(size_t)y * W + x] = res;
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------------------

@@ -117,10 +117,12 @@ hipError_t launch_wta(const PairJob& j, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------- post
+constexpr int kSpkTW = 64, kSpkTH = 16;
+
 __global__ __launch_bounds__(256) void speckle_kernel(PairJob j) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (x >= j.W || y >= j.H) return;
-    speckle_px(gmem(j.dispLR), gmem(j.spk), j.W, j.H, x, y, j.speckleMin, j.speckleDiff);
+    __shared__ int16_t tile[(kSpkTH + 4) * (kSpkTW + 4)];
+    speckle_tile<kSpkTW, kSpkTH>(gmem(j.dispLR), gmem(j.spk), j.W, j.H, blockIdx.x * kSpkTW,
+                                 blockIdx.y * kSpkTH, j.speckleMin, j.speckleDiff, tile);
 }
 
 __global__ __launch_bounds__(256) void median3x3_kernel(PairJob j, int16_t* mirror) {
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(256) void median3x3_kernel(PairJob j, int16_t* mirr
 
 hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror) {
     if (j.doSpeckle) {
-        dim3 grid((j.W + 15) / 16, (j.H + 15) / 16);
+        dim3 grid((j.W + kSpkTW - 1) / kSpkTW, (j.H + kSpkTH - 1) / kSpkTH);
         hipLaunchKernelGGL(speckle_kernel, grid, dim3(256), 0, s, j);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
