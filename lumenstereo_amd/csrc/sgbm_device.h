@@ -321,32 +321,45 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
         // prefilter_at's edge rule: column 0 takes row[1] - row[0], column W-1 row[W-1] -
         // row[W-2]. Taps outside the image stage 0 in both tiles and add nothing.
         constexpr int NQ = (2 * R + 1 + 3) / 4;
-        for (int c = tid; c < TL + TR; c += nt) {
-            const bool isL = c < TL;
-            const int gc = isL ? lbase + c : rbase + (c - TL);
-            const bool colOk = gc >= 0 && gc < W;
-            const int cl = min(max(gc - 1, 0), W - 1), cr = min(max(gc + 1, 0), W - 1);
-            const LUMEN_G uint8_t* img = gmem(isL ? j.left : j.right);
-            uint32_t a[2 * R + 1], b[2 * R + 1];
+        // Two columns per thread and pass (c and c + nt), both columns' loads in flight before
+        // the first use: a segment of up to 2 * nt columns (Dp up to 384) costs one round trip.
+        for (int cb = tid; cb < TL + TR; cb += 2 * nt) {
+            uint32_t a[2][2 * R + 1], b[2][2 * R + 1];
 #pragma unroll
-            for (int k = 0; k <= 2 * R; ++k) {
-                const size_t row = (size_t)min(max(y - R + k, 0), H - 1) * W;
-                a[k] = img[row + cr];
-                b[k] = img[row + cl];
+            for (int h = 0; h < 2; ++h) {
+                const int c = min(cb + h * nt, TL + TR - 1);  // a past-the-end column loads a real one
+                const bool isL = c < TL;
+                const int gc = isL ? lbase + c : rbase + (c - TL);
+                const int cl = min(max(gc - 1, 0), W - 1), cr = min(max(gc + 1, 0), W - 1);
+                const LUMEN_G uint8_t* img = gmem(isL ? j.left : j.right);
+#pragma unroll
+                for (int k = 0; k <= 2 * R; ++k) {
+                    const size_t row = (size_t)min(max(y - R + k, 0), H - 1) * W;
+                    a[h][k] = img[row + cr];
+                    b[h][k] = img[row + cl];
+                }
             }
-            uint32_t w[NQ];
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) w[q] = 0;
+            for (int h = 0; h < 2; ++h) {
+                const int c = cb + h * nt;
+                if (c >= TL + TR) break;
+                const bool isL = c < TL;
+                const int gc = isL ? lbase + c : rbase + (c - TL);
+                const bool colOk = gc >= 0 && gc < W;
+                uint32_t w[NQ];
 #pragma unroll
-            for (int k = 0; k <= 2 * R; ++k) {
-                const int yy = y - R + k;
-                const int g = min(cap, max(-cap, static_cast<int>(a[k]) - static_cast<int>(b[k])));
-                const uint32_t v = (colOk && yy >= 0 && yy < H) ? static_cast<uint32_t>(g + cap) : 0u;
-                w[k / 4] |= v << (8 * (k % 4));
+                for (int q = 0; q < NQ; ++q) w[q] = 0;
+#pragma unroll
+                for (int k = 0; k <= 2 * R; ++k) {
+                    const int yy = y - R + k;
+                    const int g = min(cap, max(-cap, static_cast<int>(a[h][k]) - static_cast<int>(b[h][k])));
+                    const uint32_t v = (colOk && yy >= 0 && yy < H) ? static_cast<uint32_t>(g + cap) : 0u;
+                    w[k / 4] |= v << (8 * (k % 4));
+                }
+                uint32_t* dst = isL ? lt + c * NQ : rt + (c - TL) * NQ;
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) dst[q] = w[q];
             }
-            uint32_t* dst = isL ? lt + c * NQ : rt + (c - TL) * NQ;
-#pragma unroll
-            for (int q = 0; q < NQ; ++q) dst[q] = w[q];
         }
     } else {
         uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
