@@ -44,29 +44,30 @@ __device__ __forceinline__ LUMEN_G T* gmem(T* p) {
     return (LUMEN_G T*)p;
 }
 
-// DPP controls (GFX9 encoding).
+// DPP controls (GFX9 encoding). Each names a source lane of the same row for every lane.
 constexpr int kQuadXor1 = 0xB1;        // quad_perm [1,0,3,2]
 constexpr int kQuadXor2 = 0x4E;        // quad_perm [2,3,0,1]
-constexpr int kRowShl1 = 0x101;        // lane i reads lane i+1 of its row
-constexpr int kRowShr1 = 0x111;        // lane i reads lane i-1 of its row
 constexpr int kRowRor1 = 0x121;        // lane i reads lane (i-1) mod 16 of its row
 constexpr int kRowRor15 = 0x12F;       // lane i reads lane (i+1) mod 16 of its row
 constexpr int kRowMirror = 0x140;      // lane i reads lane 15-i of its row
 constexpr int kRowHalfMirror = 0x141;  // lane i reads lane 7-i of its half row
 
+// v of the lane CTRL names. Only controls that give every lane a source are used (the ones
+// above and row_newbcast), so no lane keeps an old value and the bound_ctrl form is exact; it
+// also lets the compiler fold the move into the VALU instruction that consumes it (v_min_u32
+// with a DPP source), one instruction per butterfly stage instead of three.
 template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t v, uint32_t old) {
-    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old),
-                                                             static_cast<int>(v), CTRL, 0xF, 0xF,
-                                                             false));
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return static_cast<uint32_t>(
+        __builtin_amdgcn_mov_dpp(static_cast<int>(v), CTRL, 0xF, 0xF, true));
 }
 
 // Minimum over the 16 lanes of a DPP row, left in every lane of the row.
 __device__ __forceinline__ uint32_t row_min(uint32_t v) {
-    v = min(v, dpp<kQuadXor1>(v, v));
-    v = min(v, dpp<kQuadXor2>(v, v));
-    v = min(v, dpp<kRowHalfMirror>(v, v));
-    v = min(v, dpp<kRowMirror>(v, v));
+    v = min(v, dpp<kQuadXor1>(v));
+    v = min(v, dpp<kQuadXor2>(v));
+    v = min(v, dpp<kRowHalfMirror>(v));
+    v = min(v, dpp<kRowMirror>(v));
     return v;
 }
 
@@ -74,22 +75,22 @@ __device__ __forceinline__ uint32_t row_min(uint32_t v) {
 // constant after unrolling.
 __device__ __forceinline__ uint32_t row_bcast(uint32_t v, int u) {
     switch (u) {
-        case 0: return dpp<0x150>(v, v);
-        case 1: return dpp<0x151>(v, v);
-        case 2: return dpp<0x152>(v, v);
-        case 3: return dpp<0x153>(v, v);
-        case 4: return dpp<0x154>(v, v);
-        case 5: return dpp<0x155>(v, v);
-        case 6: return dpp<0x156>(v, v);
-        case 7: return dpp<0x157>(v, v);
-        case 8: return dpp<0x158>(v, v);
-        case 9: return dpp<0x159>(v, v);
-        case 10: return dpp<0x15A>(v, v);
-        case 11: return dpp<0x15B>(v, v);
-        case 12: return dpp<0x15C>(v, v);
-        case 13: return dpp<0x15D>(v, v);
-        case 14: return dpp<0x15E>(v, v);
-        default: return dpp<0x15F>(v, v);
+        case 0: return dpp<0x150>(v);
+        case 1: return dpp<0x151>(v);
+        case 2: return dpp<0x152>(v);
+        case 3: return dpp<0x153>(v);
+        case 4: return dpp<0x154>(v);
+        case 5: return dpp<0x155>(v);
+        case 6: return dpp<0x156>(v);
+        case 7: return dpp<0x157>(v);
+        case 8: return dpp<0x158>(v);
+        case 9: return dpp<0x159>(v);
+        case 10: return dpp<0x15A>(v);
+        case 11: return dpp<0x15B>(v);
+        case 12: return dpp<0x15C>(v);
+        case 13: return dpp<0x15D>(v);
+        case 14: return dpp<0x15E>(v);
+        default: return dpp<0x15F>(v);
     }
 }
 
@@ -675,8 +676,8 @@ __device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&d
     uint32_t prevLast[NC], nextFirst[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]), 0u);
-        nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]), 0u);
+        prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]));
+        nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]));
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -933,8 +934,8 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
     }
 #define LUMEN_WTA_COMBINE(CTRL)                   \
     {                                             \
-        const uint32_t ok = dpp<CTRL>(k1, k1);    \
-        const uint32_t om = dpp<CTRL>(m2, m2);    \
+        const uint32_t ok = dpp<CTRL>(k1);        \
+        const uint32_t om = dpp<CTRL>(m2);        \
         m2 = min(min(m2, om), max(k1, ok) >> 16); \
         k1 = min(k1, ok);                         \
     }
