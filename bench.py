@@ -44,24 +44,15 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0
 METRIC = "MDE/s + frames/s at 1080p D=128 Census; batched pairs/s and p50/p99 vs plain"
 
-SEED = 20240611
-
-# Single-pair workloads: (width, height, D, census, paths, scene disparity range, low-texture
-# fraction). The default is the metric's configuration; the others are for profiling.
-WORKLOADS = {
-    "1080p-d128-census-8": (1920, 1080, 128, True, 8, (4, 124), 0.0),
-    "480p-d64-sad-4": (640, 480, 64, False, 4, (4, 60), 0.0),
-    "2880x1988-d290-census-4": (2880, 1988, 290, True, 4, (10, 280), 0.2),
-}
-DEFAULT_WORKLOAD = "1080p-d128-census-8"
+from lumenstereo_amd.workloads import DEFAULT_SINGLE as DEFAULT_WORKLOAD  # noqa: E402
+from lumenstereo_amd.workloads import SEED  # noqa: E402
+from lumenstereo_amd.workloads import SINGLE as WORKLOADS  # noqa: E402
 
 
 def workload_params(ls, D, census, paths):
-    return ls.StereoParams(minDisparity=0, maxDisparity=D, blockSize=5, P1=200, P2=800,
-                           preFilterCap=31, uniquenessRatio=0, disp12MaxDiff=1,
-                           speckleWindowSize=0, numDirections=paths,
-                           matchingCostMode=ls.MatchingCostMode.Census if census
-                           else ls.MatchingCostMode.SAD)
+    from lumenstereo_amd import workloads
+
+    return workloads.pair_params(D, census, paths)
 
 
 def load_traffic(workload):

@@ -3,7 +3,8 @@
  *
  * This is the drop-in boundary for the reference's matcher. Everything is extern "C", takes
  * plain pointers and sizes, never throws, and returns an int status:
- *   LUMEN_OK (0), LUMEN_EINVAL (1), LUMEN_EDEVICE (2), LUMEN_ENOMEM (3).
+ *   LUMEN_OK (0), LUMEN_EINVAL (1), LUMEN_EDEVICE (2), LUMEN_ENOMEM (3), and
+ *   LUMEN_ETIMEDOUT (4) from lumen_server_wait only.
  * The text of the last error of a handle is returned by lumen_sgbm_last_error(); errors that
  * occur before a handle exists are reported by lumen_last_error().
  *
@@ -27,6 +28,7 @@ extern "C" {
 #define LUMEN_EINVAL 1
 #define LUMEN_EDEVICE 2
 #define LUMEN_ENOMEM 3
+#define LUMEN_ETIMEDOUT 4
 
 /* Largest disparity range (maxDisparity - minDisparity); replaces kSgbmMaxDisparityRange
  * (include/stereo/sgbm_limits.h:9). */
@@ -256,9 +258,12 @@ int lumen_server_destroy(lumen_server* s);
  * LUMEN_ENOMEM if its working set is larger than the whole pool. The images and the output must
  * stay valid until lumen_server_wait returns for the ticket. */
 int lumen_server_submit(lumen_server* s, const lumen_pair_desc* pair, uint64_t* ticket);
-/* Waits for a ticket (timeout_us < 0: no limit) and releases it. Returns the request's status,
- * or LUMEN_EINVAL for an unknown ticket; on timeout returns LUMEN_EDEVICE with the ticket kept.
- * info may be NULL. */
+/* Waits for a ticket (timeout_us < 0: no limit) and releases it. Returns the request's status
+ * (the ticket is then released and the request's buffers belong to the caller again), or
+ * LUMEN_EINVAL for an unknown ticket or one that another thread is already waiting on. On
+ * timeout it returns LUMEN_ETIMEDOUT and keeps the ticket: the request may still be queued or
+ * running, so its images and output must stay valid until a later wait returns its status.
+ * A ticket has at most one waiter at a time. info may be NULL. */
 int lumen_server_wait(lumen_server* s, uint64_t ticket, int64_t timeout_us,
                       lumen_request_info* info);
 /* Waits until no request is queued or running. */

@@ -19,6 +19,7 @@ import ctypes
 import dataclasses
 import enum
 import os
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -47,7 +48,7 @@ LIB_PATH = Path(__file__).resolve().parent / "liblumen_sgbm.so"
 # include/stereo/sgbm_limits.h:9
 kSgbmMaxDisparityRange = 512
 
-LUMEN_OK, LUMEN_EINVAL, LUMEN_EDEVICE, LUMEN_ENOMEM = 0, 1, 2, 3
+LUMEN_OK, LUMEN_EINVAL, LUMEN_EDEVICE, LUMEN_ENOMEM, LUMEN_ETIMEDOUT = 0, 1, 2, 3, 4
 
 STAGE = {
     "cost": 0,
@@ -564,7 +565,11 @@ class ServeSGBM:
         rc = self._lib.lumen_server_create(ctypes.byref(cfg), ctypes.byref(self._h))
         if rc != LUMEN_OK:
             raise GpuException(self._lib.lumen_server_last_error(None).decode(), rc)
+        # ticket -> (left, right, out) of host requests: the server reads and writes these
+        # buffers until a wait returns the ticket's status, so they live here until then.
         self._live = {}
+        self._waiting = set()
+        self._lock = threading.Lock()
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -607,10 +612,25 @@ class ServeSGBM:
         return t.value
 
     def wait(self, ticket: int, timeout_s: float | None = None):
-        """Waits for a ticket; returns (disparity or None for device requests, info dict)."""
+        """Waits for a ticket; returns (disparity or None for device requests, info dict).
+
+        Raises TimeoutError when the request is not done within timeout_s: the ticket stays
+        valid (the request may still be queued or running, so its buffers are kept) and may be
+        waited on again. A ticket takes one waiter at a time."""
+        with self._lock:
+            if ticket in self._waiting:
+                raise StereoException("ticket already has a waiter")
+            self._waiting.add(ticket)
         info = CRequestInfo()
         tmo = -1 if timeout_s is None else int(timeout_s * 1e6)
-        rc = self._lib.lumen_server_wait(self._h, ticket, tmo, ctypes.byref(info))
+        try:
+            rc = self._lib.lumen_server_wait(self._h, ticket, tmo, ctypes.byref(info))
+        finally:
+            with self._lock:
+                self._waiting.discard(ticket)
+        if rc == LUMEN_ETIMEDOUT:
+            raise TimeoutError(f"ticket {ticket} not done after {timeout_s} s")
+        # Any other status means the server released the ticket: its buffers are free again.
         held = self._live.pop(ticket, None)
         if rc != LUMEN_OK:
             self._raise(rc)

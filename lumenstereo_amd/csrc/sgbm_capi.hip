@@ -88,6 +88,7 @@ constexpr int kNumEvents = 6;  // start, uploaded, cost, paths, wta, post
 constexpr int kRing = 64;
 
 struct Geometry {
+    lumen_stereo_params params{};  // of the last prepared compute (the dump hook's frame)
     int W = 0, H = 0, D = 0, Dp = 0, m = 0, npaths = 0, costScale = 0;
     size_t pixels() const { return (size_t)W * H; }
     size_t volume() const { return pixels() * Dp; }
@@ -107,7 +108,7 @@ struct lumen_sgbm {
     Geometry geo;
     bool haveResult = false;
     DevBuf left, right, cost, paths, dispL, dispR, dispLR, wtaInt, spk, out,
-        sum, trash;
+        sum, trash, dumpTmp;
     PinnedBuf stageIn, stageOut;
     bool hasSchedule = false;  // lumen_sgbm_set_schedule override
     Schedule schedule{};
@@ -142,8 +143,8 @@ struct lumen_sgbm {
                     if (ev[i][j]) (void)hipEventDestroy(ev[i][j]);
             (void)hipStreamDestroy(stream);
         }
-        DevBuf* all[] = {&left, &right, &cost, &paths, &dispL,
-                         &dispR, &dispLR, &wtaInt, &spk, &out, &sum, &trash};
+        DevBuf* all[] = {&left, &right, &cost, &paths, &dispL,   &dispR, &dispLR,
+                         &wtaInt, &spk, &out, &sum, &trash, &dumpTmp};
         for (DevBuf* b : all) b->release();
         stageIn.release();
         stageOut.release();
@@ -169,6 +170,7 @@ int prepare(lumen_sgbm* h, int W, int H) {
     if (rc) return rc;
     Geometry& g = h->geo;
     const lumen_stereo_params& p = h->params;
+    g.params = p;
     g.W = W;
     g.H = H;
     g.m = p.minDisparity;
@@ -202,13 +204,11 @@ int prepare(lumen_sgbm* h, int W, int H) {
     return LUMEN_OK;
 }
 
-// Enqueues every stage after the upload; images are already in h->left / h->right or in the
-// caller's device buffers.
-int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dout, int slot) {
+// The PairJob of the last prepared frame (its parameters and size) over the handle's buffers.
+PairJob make_job(const lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dout) {
     const Geometry& g = h->geo;
-    hipStream_t s = h->stream;
     PairJob j{};
-    fill_job(j, h->params, g.W, g.H);
+    fill_job(j, g.params, g.W, g.H);
     if (h->hasSchedule) j.sched = h->schedule;  // checked by validate()
     j.left = dl;
     j.right = dr;
@@ -220,6 +220,14 @@ int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dou
     j.wtaInt = h->wtaInt.as<int16_t>();
     j.spk = h->spk.as<int16_t>();
     j.out = dout;
+    return j;
+}
+
+// Enqueues every stage after the upload; images are already in h->left / h->right or in the
+// caller's device buffers.
+int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dout, int slot) {
+    hipStream_t s = h->stream;
+    const PairJob j = make_job(h, dl, dr, dout);
     hipError_t e = launch_cost(j, s);
     if (e != hipSuccess) return h->hip_fail(e, "cost launch");
     e = hipEventRecord(h->ev[slot][2], s);
@@ -499,11 +507,20 @@ int lumen_sgbm_dump_stage(lumen_sgbm* h, int stage, int path, void* host, size_t
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return h->hip_fail(e, "sync");
     auto strip = [&](const void* dev, size_t elem) -> int {
-        // Padded [pixels][Dp] -> host [pixels][D].
+        // Padded [pixels][Dp] -> host [pixels][D]: stripped on the device, then one copy.
         const size_t need = n * g.D * elem;
         if (bytes < need) return h->fail(LUMEN_EINVAL, "dump buffer too small");
-        hipError_t err = hipMemcpy2D(host, g.D * elem, dev, g.Dp * elem, g.D * elem, n,
-                                     hipMemcpyDeviceToHost);
+        const void* src = dev;
+        if (g.D != g.Dp) {
+            hipError_t err = h->dumpTmp.ensure(need);
+            if (err != hipSuccess) return h->hip_fail(err, "dump buffer");
+            err = launch_strip(dev, h->dumpTmp.ptr, n, g.D, g.Dp, static_cast<int>(elem), s);
+            if (err != hipSuccess) return h->hip_fail(err, "dump strip");
+            err = hipStreamSynchronize(s);
+            if (err != hipSuccess) return h->hip_fail(err, "dump strip");
+            src = h->dumpTmp.ptr;
+        }
+        hipError_t err = hipMemcpy(host, src, need, hipMemcpyDeviceToHost);
         if (err != hipSuccess) return h->hip_fail(err, "dump copy");
         return LUMEN_OK;
     };
@@ -519,10 +536,12 @@ int lumen_sgbm_dump_stage(lumen_sgbm* h, int stage, int path, void* host, size_t
             if (path < 0 || path >= g.npaths) return h->fail(LUMEN_EINVAL, "path index out of range");
             return strip(h->paths.as<uint16_t>() + (size_t)path * g.volume(), 2);
         case LUMEN_STAGE_SUM: {
+            // The sum the WTA works on lives only in registers: the WTA rows run again (same
+            // inputs, same maps written) in the instantiation that also stores it.
             e = h->sum.ensure(g.volume() * 2);
             if (e != hipSuccess) return h->hip_fail(e, "sum buffer");
-            e = launch_path_sum(h->paths.as<uint16_t>(), g.volume(), g.npaths, h->sum.as<uint16_t>(),
-                                g.volume(), s);
+            const PairJob j = make_job(h, nullptr, nullptr, h->out.as<int16_t>());
+            e = launch_wta(j, s, h->sum.as<uint16_t>());
             if (e != hipSuccess) return h->hip_fail(e, "sum launch");
             e = hipStreamSynchronize(s);
             if (e != hipSuccess) return h->hip_fail(e, "sum");
@@ -533,7 +552,7 @@ int lumen_sgbm_dump_stage(lumen_sgbm* h, int stage, int path, void* host, size_t
         case LUMEN_STAGE_DISP_RIGHT: return plain(h->dispR.ptr);
         case LUMEN_STAGE_AFTER_LR: return plain(h->dispLR.ptr);
         case LUMEN_STAGE_AFTER_SPECKLE:
-            return plain(h->params.speckleWindowSize > 0 ? h->spk.ptr : h->dispLR.ptr);
+            return plain(g.params.speckleWindowSize > 0 ? h->spk.ptr : h->dispLR.ptr);
         case LUMEN_STAGE_OUTPUT: return plain(h->out.ptr);
         default: return h->fail(LUMEN_EINVAL, "unknown stage");
     }

@@ -56,6 +56,13 @@ constexpr int kRowHalfMirror = 0x141;  // lane i reads lane 7-i of its half row
 // above and row_newbcast), so no lane keeps an old value and the bound_ctrl form is exact; it
 // also lets the compiler fold the move into the VALU instruction that consumes it (v_min_u32
 // with a DPP source), one instruction per butterfly stage instead of three.
+//
+// Contract: every caller keeps whole 16-lane rows active (no divergent branch around a DPP
+// use that disables part of a row). A DPP read of a disabled source lane returns 0 in the
+// bound_ctrl form instead of that lane's value, which would silently change a minimum. The
+// aggregation meets this by running finished or missing scanlines on clamped inputs with their
+// stores redirected, never by branching lanes off; the WTA hands each pixel to a whole row, so
+// its loop conditions are uniform within a row.
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
     return static_cast<uint32_t>(
@@ -354,7 +361,11 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
                 for (int k = 0; k <= 2 * R; ++k) {
                     const int yy = y - R + k;
                     const int g = min(cap, max(-cap, static_cast<int>(a[h][k]) - static_cast<int>(b[h][k])));
-                    const uint32_t v = (colOk && yy >= 0 && yy < H) ? static_cast<uint32_t>(g + cap) : 0u;
+                    // The prefiltered value is stored as uint8 (src/sgbm_kernels.cu:58): for
+                    // caps of 128 and more g + cap can exceed 255 and wraps mod 256, and must
+                    // not spill into the next row's byte.
+                    const uint32_t v =
+                        (colOk && yy >= 0 && yy < H) ? (static_cast<uint32_t>(g + cap) & 0xFFu) : 0u;
                     w[k / 4] |= v << (8 * (k % 4));
                 }
                 uint32_t* dst = isL ? lt + c * NQ : rt + (c - TL) * NQ;
@@ -898,12 +909,14 @@ __device__ __forceinline__ uint32_t pick32(const uint32_t (&v)[K / 2], int slot)
 
 // One pixel of a row for the DPP row of lane l: sums the NR path-cost vectors in `raw`
 // (mod 2^16), finds the left winner and second minimum, writes the left maps, and offers every
-// (pixel, d) to its right pixel in `rbest`.
-template <int K, bool PAD, int NR>
+// (pixel, d) to its right pixel in `rbest`. DUMP (the parity hook's instantiation) also stores
+// the sum vector to sumOut ([H][W][Dp], the cost volume's layout), so the sum the WTA works on
+// is the one compared.
+template <int K, bool PAD, int NR, bool DUMP = false>
 __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t rowPix,
                                           const uint32_t (&raw)[NR][K / 2],
                                           const uint32_t (&padm)[K / 2], int16_t* sdl,
-                                          unsigned long long* rbest) {
+                                          unsigned long long* rbest, uint16_t* sumOut) {
     constexpr int Q = slice_q<K>();
     const int D = j.D, m = j.m;
     u16x2 S[K / 2];
@@ -913,6 +926,7 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
 #pragma unroll
         for (int r = 1; r < NR; ++r) S[q] += as16(raw[r][q]);
     }
+    if constexpr (DUMP) store_path<K>(gmem(sumOut) + (rowPix + x) * j.Dp + l * Q, S);
     if constexpr (PAD) {
 #pragma unroll
         for (int q = 0; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
@@ -1016,8 +1030,9 @@ __device__ __forceinline__ void wta_load(const PairJob& j, size_t pix, int l,
 // compile-time count puts every plane's load of a pixel in flight before the first add, and
 // while a pixel's path costs fit 32 registers the pixel loop is software-pipelined (the next
 // pixel's loads are issued before the current pixel is processed).
-template <int K, bool PAD, int NP>
-__device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
+template <int K, bool PAD, int NP, bool DUMP = false>
+__device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest,
+                        uint16_t* sumOut = nullptr) {
     static_assert(NP == 4 || NP == 8, "the path count is 4 or 8");
     // rbest[xr + xr/Q]: running right-WTA key of right pixel xr. One padding entry every Q
     // entries spreads the 16 lanes of a row (whose candidates are Q right pixels apart) over
@@ -1045,16 +1060,16 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest) {
         for (; x < W; x += 2 * ngroups) {
             const int x1 = x + ngroups, x2 = x1 + ngroups;
             if (x1 < W) wta_load<K, NP>(j, rowPix + x1, l, bufB);
-            wta_pixel<K, PAD, NP>(j, x, l, rowPix, bufA, padm, sdl, rbest);
+            wta_pixel<K, PAD, NP, DUMP>(j, x, l, rowPix, bufA, padm, sdl, rbest, sumOut);
             if (x2 < W) wta_load<K, NP>(j, rowPix + x2, l, bufA);
-            if (x1 < W) wta_pixel<K, PAD, NP>(j, x1, l, rowPix, bufB, padm, sdl, rbest);
+            if (x1 < W) wta_pixel<K, PAD, NP, DUMP>(j, x1, l, rowPix, bufB, padm, sdl, rbest, sumOut);
         }
     } else {
         // Wide slices: a second buffer would cost more occupancy than the overlap gains.
         uint32_t buf[NP][K / 2];
         for (int x = g; x < W; x += ngroups) {
             wta_load<K, NP>(j, rowPix + x, l, buf);
-            wta_pixel<K, PAD, NP>(j, x, l, rowPix, buf, padm, sdl, rbest);
+            wta_pixel<K, PAD, NP, DUMP>(j, x, l, rowPix, buf, padm, sdl, rbest, sumOut);
         }
     }
     __syncthreads();

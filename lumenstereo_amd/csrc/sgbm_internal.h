@@ -67,11 +67,13 @@ struct PairJob {
 // Per-pair launches (sgbm_kernels.hip).
 hipError_t launch_cost(const PairJob& j, hipStream_t s);
 hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s);
-hipError_t launch_wta(const PairJob& j, hipStream_t s);
+// sumOut (may be null): the parity hook's buffer for the fused path sum ([H][W][Dp]).
+hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut = nullptr);
 // mirror (may be null): a second output buffer the median also writes.
 hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror = nullptr);
-hipError_t launch_path_sum(const uint16_t* L, size_t pathStride, int npaths, uint16_t* S,
-                           size_t n, hipStream_t s);
+// [n][Dp] -> [n][D] copy of a padded volume of 1- or 2-byte entries (parity dumps).
+hipError_t launch_strip(const void* src, void* dst, size_t n, int D, int Dp, int elem,
+                        hipStream_t s);
 // Number of scanlines of SGM path `path` (0..7) over a W x H image.
 int scanline_count(int path, int W, int H);
 

@@ -70,15 +70,29 @@ __global__ __launch_bounds__(NT) void wta_row_kernel(PairJob j) {
     wta_row<K, PAD, NP>(j, blockIdx.x, rbest);
 }
 
+// The parity hook's form: the same rows, also storing each pixel's path sum to sumOut.
+template <int K, bool PAD, int NP>
+__global__ __launch_bounds__(256) void wta_row_dump_kernel(PairJob j, uint16_t* sumOut) {
+    extern __shared__ unsigned long long rbest[];
+    wta_row<K, PAD, NP, true>(j, blockIdx.x, rbest, sumOut);
+}
+
 // Frames with fewer rows than kWtaWideRows (about two per CU) give each row 512 threads, so a
 // row takes half the sequential steps; the arithmetic and the atomics' outcome do not depend
 // on the block size.
 constexpr int kWtaWideRows = 512;
 
 template <int K, int NP>
-static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s) {
+static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s, uint16_t* sumOut) {
     const size_t lds = wta_lds_bytes<K>(j.W);
     const bool pad = j.D != j.Dp;
+    if (sumOut) {
+        if (pad)
+            hipLaunchKernelGGL((wta_row_dump_kernel<K, true, NP>), dim3(j.H), dim3(256), lds, s, j, sumOut);
+        else
+            hipLaunchKernelGGL((wta_row_dump_kernel<K, false, NP>), dim3(j.H), dim3(256), lds, s, j, sumOut);
+        return hipGetLastError();
+    }
     if constexpr (K <= 8) {
         if (j.H < kWtaWideRows) {
             if (pad)
@@ -97,21 +111,21 @@ static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s) {
 }
 
 template <int K>
-static hipError_t launch_wta_k(const PairJob& j, hipStream_t s) {
-    return j.npaths == 8 ? launch_wta_kn<K, 8>(j, s) : launch_wta_kn<K, 4>(j, s);
+static hipError_t launch_wta_k(const PairJob& j, hipStream_t s, uint16_t* sumOut) {
+    return j.npaths == 8 ? launch_wta_kn<K, 8>(j, s, sumOut) : launch_wta_kn<K, 4>(j, s, sumOut);
 }
 
-hipError_t launch_wta(const PairJob& j, hipStream_t s) {
+hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut) {
     if (j.W > kMaxWidth) return hipErrorInvalidValue;
     switch (j.K) {
-        case 4: return launch_wta_k<4>(j, s);
-        case 8: return launch_wta_k<8>(j, s);
-        case 12: return launch_wta_k<12>(j, s);
-        case 16: return launch_wta_k<16>(j, s);
-        case 20: return launch_wta_k<20>(j, s);
-        case 24: return launch_wta_k<24>(j, s);
-        case 28: return launch_wta_k<28>(j, s);
-        case 32: return launch_wta_k<32>(j, s);
+        case 4: return launch_wta_k<4>(j, s, sumOut);
+        case 8: return launch_wta_k<8>(j, s, sumOut);
+        case 12: return launch_wta_k<12>(j, s, sumOut);
+        case 16: return launch_wta_k<16>(j, s, sumOut);
+        case 20: return launch_wta_k<20>(j, s, sumOut);
+        case 24: return launch_wta_k<24>(j, s, sumOut);
+        case 28: return launch_wta_k<28>(j, s, sumOut);
+        case 32: return launch_wta_k<32>(j, s, sumOut);
         default: return hipErrorInvalidValue;
     }
 }
@@ -143,21 +157,27 @@ hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror) {
     return hipGetLastError();
 }
 
-// Sum of all path costs mod 2^16 into a padded volume (parity dumps only).
-__global__ __launch_bounds__(256) void path_sum_kernel(const uint16_t* __restrict__ L,
-                                                       size_t pathStride, int npaths,
-                                                       uint16_t* __restrict__ S, size_t n) {
+// [n][Dp] -> [n][D] copy of a padded volume of ELEM-byte entries (parity dumps only): the
+// dump then leaves the device as one contiguous copy.
+template <typename T>
+__global__ __launch_bounds__(256) void strip_pad_kernel(const T* __restrict__ src, T* __restrict__ dst,
+                                                        size_t total, int D, int Dp) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    uint32_t acc = 0;
-    for (int r = 0; r < npaths; ++r) acc += L[(size_t)r * pathStride + i];
-    S[i] = static_cast<uint16_t>(acc & 0xFFFFu);
+    if (i >= total) return;
+    const size_t p = i / D;
+    dst[i] = src[p * Dp + (i - p * D)];
 }
 
-hipError_t launch_path_sum(const uint16_t* L, size_t pathStride, int npaths, uint16_t* S,
-                           size_t n, hipStream_t s) {
-    hipLaunchKernelGGL(path_sum_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, L,
-                       pathStride, npaths, S, n);
+hipError_t launch_strip(const void* src, void* dst, size_t n, int D, int Dp, int elem,
+                        hipStream_t s) {
+    const size_t total = n * D;
+    const dim3 grid((unsigned)((total + 255) / 256));
+    if (elem == 1)
+        hipLaunchKernelGGL(strip_pad_kernel<uint8_t>, grid, dim3(256), 0, s,
+                           static_cast<const uint8_t*>(src), static_cast<uint8_t*>(dst), total, D, Dp);
+    else
+        hipLaunchKernelGGL(strip_pad_kernel<uint16_t>, grid, dim3(256), 0, s,
+                           static_cast<const uint16_t*>(src), static_cast<uint16_t*>(dst), total, D, Dp);
     return hipGetLastError();
 }
 

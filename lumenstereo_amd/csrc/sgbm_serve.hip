@@ -65,6 +65,7 @@ struct Request {
     std::string err;
     lumen_request_info info{};
     bool done = false;
+    bool waiting = false;  // a thread is in lumen_server_wait for this ticket
 };
 
 struct Lane {
@@ -440,12 +441,20 @@ int lumen_server_wait(lumen_server* s, uint64_t ticket, int64_t timeout_us,
         return LUMEN_EINVAL;
     }
     Request* r = it->second.get();
+    // One waiter per ticket: the waiter that sees the request done erases it, so a second
+    // waiter would be left holding a freed request.
+    if (r->waiting) {
+        g_serve_error = "ticket already has a waiter";
+        return LUMEN_EINVAL;
+    }
+    r->waiting = true;
     auto isDone = [&] { return r->done; };
     if (timeout_us < 0) {
         s->doneCv.wait(lk, isDone);
     } else if (!s->doneCv.wait_for(lk, std::chrono::microseconds(timeout_us), isDone)) {
+        r->waiting = false;
         g_serve_error = "timed out";
-        return LUMEN_EDEVICE;
+        return LUMEN_ETIMEDOUT;
     }
     if (info) *info = r->info;
     const int status = r->status;

@@ -56,7 +56,8 @@ def lib() -> ctypes.CDLL:
             "oracle_median": [vp, vp, ci, ci],
             "oracle_sgbm": [ctypes.POINTER(OracleParams), vp, vp, ci, ci, vp,
                             ctypes.POINTER(OracleDumps)],
-            "oracle_sgbm_mt": [ctypes.POINTER(OracleParams), vp, vp, ci, ci, vp, ci],
+            "oracle_sgbm_mt": [ctypes.POINTER(OracleParams), vp, vp, ci, ci, vp, ci,
+                               ctypes.POINTER(OracleDumps)],
             # scene_oracle.h
             "oracle_remap": [vp, vp, vp, vp, ci, ci, ci],
             "oracle_disparity_to_depth": [vp, vp, ci, ci, ctypes.c_float, ctypes.c_float,
@@ -173,8 +174,9 @@ def median(d):
     return out
 
 
-def sgbm(params, left, right, dumps: bool = False):
-    """Full matcher. Returns the output, or (output, dict of stage arrays) when dumps=True."""
+def sgbm(params, left, right, dumps: bool = False, threads: int = 1):
+    """Full matcher. Returns the output, or (output, dict of stage arrays) when dumps=True.
+    threads > 1 runs the threaded form (identical results, for full-size frames)."""
     left, right = _c(left, np.uint8), _c(right, np.uint8)
     H, W = left.shape
     ps = params_struct(params)
@@ -195,22 +197,20 @@ def sgbm(params, left, right, dumps: bool = False):
             "after_speckle": np.empty((H, W), np.int16),
         }
         d = OracleDumps(*[_p(st[n]) for n, _ in OracleDumps._fields_])
-    rc = lib().oracle_sgbm(ctypes.byref(ps), _p(left), _p(right), W, H, _p(out),
-                           ctypes.byref(d) if d is not None else None)
+    dref = ctypes.byref(d) if d is not None else None
+    if threads > 1:
+        rc = lib().oracle_sgbm_mt(ctypes.byref(ps), _p(left), _p(right), W, H, _p(out),
+                                  int(threads), dref)
+    else:
+        rc = lib().oracle_sgbm(ctypes.byref(ps), _p(left), _p(right), W, H, _p(out), dref)
     if rc != 0:
         raise ValueError("oracle_sgbm rejected the arguments")
     return (out, st) if dumps else out
 
 
 def sgbm_mt(params, left, right, nthreads):
-    left, right = _c(left, np.uint8), _c(right, np.uint8)
-    H, W = left.shape
-    ps = params_struct(params)
-    out = np.empty((H, W), np.int16)
-    rc = lib().oracle_sgbm_mt(ctypes.byref(ps), _p(left), _p(right), W, H, _p(out), nthreads)
-    if rc != 0:
-        raise ValueError("oracle_sgbm_mt rejected the arguments")
-    return out
+    """The threaded form without dumps (the reported CPU baseline)."""
+    return sgbm(params, left, right, dumps=False, threads=int(nthreads))
 
 
 def remap(src, mapx, mapy):

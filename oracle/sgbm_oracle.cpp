@@ -8,6 +8,7 @@
 #include "sgbm_oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -102,11 +103,11 @@ void oracle_census(const uint8_t* img, int W, int H, uint32_t* out) {
 }
 
 /* src/sgbm_kernels.cu:172-212 (the naive kernel; the shared-memory kernel :70-167 gives the
- * same values because its right tile always holds the addressed columns). */
-void oracle_cost_sad(const uint8_t* fl, const uint8_t* fr, int W, int H, int D, int m,
-                     int blockSize, uint8_t* cost) {
+ * same values because its right tile always holds the addressed columns). Rows y0..y1-1. */
+static void cost_sad_rows(const uint8_t* fl, const uint8_t* fr, int W, int H, int D, int m,
+                          int blockSize, uint8_t* cost, int y0, int y1) {
     const int r = blockSize / 2;
-    for (int y = 0; y < H; ++y)
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < W; ++x)
             for (int d = 0; d < D; ++d) {
                 const int a = m + d;
@@ -125,10 +126,15 @@ void oracle_cost_sad(const uint8_t* fl, const uint8_t* fr, int W, int H, int D, 
             }
 }
 
-/* src/sgbm_kernels.cu:250-286. */
-void oracle_cost_census(const uint32_t* cl, const uint32_t* cr, int W, int H, int D, int m,
-                        uint8_t* cost) {
-    for (int y = 0; y < H; ++y)
+void oracle_cost_sad(const uint8_t* fl, const uint8_t* fr, int W, int H, int D, int m,
+                     int blockSize, uint8_t* cost) {
+    cost_sad_rows(fl, fr, W, H, D, m, blockSize, cost, 0, H);
+}
+
+/* src/sgbm_kernels.cu:250-286. Rows y0..y1-1. */
+static void cost_census_rows(const uint32_t* cl, const uint32_t* cr, int W, int H, int D, int m,
+                             uint8_t* cost, int y0, int y1) {
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < W; ++x) {
             uint8_t* out = cost + (static_cast<size_t>(y) * W + x) * D;
             const bool leftOk = x >= 2 && x < W - 2 && y >= 2 && y < H - 2;
@@ -143,6 +149,11 @@ void oracle_cost_census(const uint32_t* cl, const uint32_t* cr, int W, int H, in
                 out[d] = static_cast<uint8_t>((h * 255 + 12) / 24);
             }
         }
+}
+
+void oracle_cost_census(const uint32_t* cl, const uint32_t* cr, int W, int H, int D, int m,
+                        uint8_t* cost) {
+    cost_census_rows(cl, cr, W, H, D, m, cost, 0, H);
 }
 
 /* src/sgbm_gpu.cu:312-317: SAD uses blockSize^2, every other mode the census formula. */
@@ -197,10 +208,11 @@ void oracle_aggregate_path(const uint8_t* cost, const uint8_t* img, int W, int H
 }
 
 /* src/sgbm_kernels.cu:748-805: first minimum with strict '<' from 65535, second minimum of
- * the multiset, uniqueness in uint32 arithmetic, parabola for 0 < bestD < D-1. */
-void oracle_wta_left(const uint16_t* S, int W, int H, int D, int m, int uniquenessRatio,
-                     int16_t* disp, int16_t* wta_int) {
-    for (size_t p = 0; p < static_cast<size_t>(W) * H; ++p) {
+ * the multiset, uniqueness in uint32 arithmetic, parabola for 0 < bestD < D-1. Pixels
+ * p0..p1-1. */
+static void wta_left_pixels(const uint16_t* S, int D, int m, int uniquenessRatio, int16_t* disp,
+                            int16_t* wta_int, size_t p0, size_t p1) {
+    for (size_t p = p0; p < p1; ++p) {
         const uint16_t* s = S + p * D;
         uint16_t best = 65535, second = 65535;
         int bestD = 0;
@@ -228,10 +240,16 @@ void oracle_wta_left(const uint16_t* S, int W, int H, int D, int m, int uniquene
     }
 }
 
+void oracle_wta_left(const uint16_t* S, int W, int H, int D, int m, int uniquenessRatio,
+                     int16_t* disp, int16_t* wta_int) {
+    wta_left_pixels(S, D, m, uniquenessRatio, disp, wta_int, 0, static_cast<size_t>(W) * H);
+}
+
 /* src/sgbm_kernels.cu:807-856: right pixel x scans S(y, x+m+d, d) while x+m+d < W; the
- * parabola uses the three costs of that same left pixel. Requires m >= 0. */
-void oracle_wta_right(const uint16_t* S, int W, int H, int D, int m, int16_t* dispR) {
-    for (int y = 0; y < H; ++y)
+ * parabola uses the three costs of that same left pixel. Requires m >= 0. Rows y0..y1-1. */
+static void wta_right_rows(const uint16_t* S, int W, int D, int m, int16_t* dispR, int y0,
+                           int y1) {
+    for (int y = y0; y < y1; ++y)
         for (int x = 0; x < W; ++x) {
             uint16_t best = 65535;
             int bestD = 0;
@@ -254,6 +272,10 @@ void oracle_wta_right(const uint16_t* S, int W, int H, int D, int m, int16_t* di
             }
             dispR[static_cast<size_t>(y) * W + x] = f2i16((sub + static_cast<float>(m)) * 16.0f);
         }
+}
+
+void oracle_wta_right(const uint16_t* S, int W, int H, int D, int m, int16_t* dispR) {
+    wta_right_rows(S, W, D, m, dispR, 0, H);
 }
 
 /* src/sgbm_kernels.cu:862-890 (in place; each pixel reads only itself and dispR). */
@@ -321,11 +343,33 @@ void oracle_median(const int16_t* in, int16_t* out, int W, int H) {
 
 namespace {
 
+/* Runs fn(lo, hi) over [0, n) split into contiguous bands on up to nthreads threads. */
+template <typename F>
+void parallel_bands(size_t n, int nthreads, F fn) {
+    if (nthreads <= 1 || n < 2) {
+        fn(size_t(0), n);
+        return;
+    }
+    const size_t parts = std::min<size_t>(static_cast<size_t>(nthreads) * 4, n);
+    std::vector<std::thread> pool;
+    std::atomic<size_t> next{0};
+    for (int t = 0; t < nthreads; ++t)
+        pool.emplace_back([&] {
+            for (size_t k; (k = next.fetch_add(1)) < parts;)
+                fn(n * k / parts, n * (k + 1) / parts);
+        });
+    for (auto& th : pool) th.join();
+}
+
+/* Stage storage: the caller's dump buffers where given, otherwise owned vectors. */
 struct Pipeline {
     const oracle_params* p;
-    int W, H, D, m, npaths, cs;
-    std::vector<uint8_t> cost;
-    std::vector<uint16_t> paths;
+    int W, H, D, m, npaths, cs, threads;
+    std::vector<uint8_t> ownCost;
+    std::vector<uint16_t> ownPaths, ownSum;
+    uint8_t* cost = nullptr;
+    uint16_t* paths = nullptr;
+    uint16_t* sum = nullptr;
 };
 
 /* Validation of src/sgbm_gpu.cu:148-166 (messages are the product's business). Negative
@@ -348,35 +392,53 @@ void cost_stage(Pipeline& st, const uint8_t* left, const uint8_t* right) {
         std::vector<uint32_t> cl(n), cr(n);
         oracle_census(left, st.W, st.H, cl.data());
         oracle_census(right, st.W, st.H, cr.data());
-        oracle_cost_census(cl.data(), cr.data(), st.W, st.H, st.D, st.m, st.cost.data());
+        parallel_bands(st.H, st.threads, [&](size_t y0, size_t y1) {
+            cost_census_rows(cl.data(), cr.data(), st.W, st.H, st.D, st.m, st.cost,
+                             static_cast<int>(y0), static_cast<int>(y1));
+        });
     } else {
         std::vector<uint8_t> fl(n), fr(n);
         oracle_prefilter(left, st.W, st.H, st.p->preFilterCap, fl.data());
         oracle_prefilter(right, st.W, st.H, st.p->preFilterCap, fr.data());
-        oracle_cost_sad(fl.data(), fr.data(), st.W, st.H, st.D, st.m, st.p->blockSize,
-                        st.cost.data());
+        parallel_bands(st.H, st.threads, [&](size_t y0, size_t y1) {
+            cost_sad_rows(fl.data(), fr.data(), st.W, st.H, st.D, st.m, st.p->blockSize,
+                          st.cost, static_cast<int>(y0), static_cast<int>(y1));
+        });
     }
 }
 
-/* WTA and post-processing of src/sgbm_gpu.cu:432-515. */
-void finish(Pipeline& st, const uint8_t* left, int16_t* out, const oracle_dumps* dumps) {
-    (void)left;
+/* The paths of src/sgbm_gpu.cu:321-421, each independent of the others (one per thread). */
+void path_stage(Pipeline& st, const uint8_t* left) {
+    const size_t vol = static_cast<size_t>(st.W) * st.H * st.D;
+    parallel_bands(st.npaths, std::min(st.threads, st.npaths), [&](size_t r0, size_t r1) {
+        for (size_t r = r0; r < r1; ++r)
+            oracle_aggregate_path(st.cost, left, st.W, st.H, st.D, st.p->P1, st.p->P2, st.cs,
+                                  static_cast<int>(r), st.paths + r * vol);
+    });
+}
+
+/* Path sum (the uint16 accumulation of aggCost, src/sgbm_gpu.cu:321-421), WTA and
+ * post-processing of src/sgbm_gpu.cu:432-515. */
+void finish(Pipeline& st, int16_t* out, const oracle_dumps* dumps) {
     const size_t n = static_cast<size_t>(st.W) * st.H;
     const size_t vol = n * st.D;
-    std::vector<uint16_t> S(vol, 0);
-    for (int r = 0; r < st.npaths; ++r) {
-        const uint16_t* Lr = st.paths.data() + static_cast<size_t>(r) * vol;
-        for (size_t i = 0; i < vol; ++i) S[i] = t16(int(S[i]) + int(Lr[i]));
-    }
-    std::vector<int16_t> dl(n), dr(n), tmp(n);
-    oracle_wta_left(S.data(), st.W, st.H, st.D, st.m, st.p->uniquenessRatio, dl.data(),
-                    dumps ? dumps->wta_int : nullptr);
-    oracle_wta_right(S.data(), st.W, st.H, st.D, st.m, dr.data());
+    uint16_t* S = st.sum;
+    parallel_bands(vol, st.threads, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; ++i) {
+            int acc = 0;
+            for (int r = 0; r < st.npaths; ++r) acc = t16(acc + int(st.paths[r * vol + i]));
+            S[i] = static_cast<uint16_t>(acc);
+        }
+    });
+    std::vector<int16_t> dl(n), dr(n);
+    int16_t* wi = dumps ? dumps->wta_int : nullptr;
+    parallel_bands(n, st.threads, [&](size_t p0, size_t p1) {
+        wta_left_pixels(S, st.D, st.m, st.p->uniquenessRatio, dl.data(), wi, p0, p1);
+    });
+    parallel_bands(st.H, st.threads, [&](size_t y0, size_t y1) {
+        wta_right_rows(S, st.W, st.D, st.m, dr.data(), static_cast<int>(y0), static_cast<int>(y1));
+    });
     if (dumps) {
-        if (dumps->cost) std::memcpy(dumps->cost, st.cost.data(), vol);
-        if (dumps->paths)
-            std::memcpy(dumps->paths, st.paths.data(), st.paths.size() * sizeof(uint16_t));
-        if (dumps->sum) std::memcpy(dumps->sum, S.data(), vol * sizeof(uint16_t));
         if (dumps->disp_left) std::memcpy(dumps->disp_left, dl.data(), n * 2);
         if (dumps->disp_right) std::memcpy(dumps->disp_right, dr.data(), n * 2);
     }
@@ -390,7 +452,8 @@ void finish(Pipeline& st, const uint8_t* left, int16_t* out, const oracle_dumps*
     oracle_median(dl.data(), out, st.W, st.H);
 }
 
-bool setup(Pipeline& st, const oracle_params* p, int W, int H) {
+void setup(Pipeline& st, const oracle_params* p, int W, int H, int threads,
+           const oracle_dumps* dumps) {
     st.p = p;
     st.W = W;
     st.H = H;
@@ -398,10 +461,23 @@ bool setup(Pipeline& st, const oracle_params* p, int W, int H) {
     st.D = p->maxDisparity - p->minDisparity;
     st.npaths = oracle_num_paths(p->numDirections);
     st.cs = oracle_cost_scale(p->matchingCostMode, p->blockSize, p->preFilterCap);
+    st.threads = std::max(1, threads);
     const size_t vol = static_cast<size_t>(W) * H * st.D;
-    st.cost.assign(vol, 0);
-    st.paths.assign(vol * st.npaths, 0);
-    return true;
+    st.cost = dumps && dumps->cost ? dumps->cost : (st.ownCost.resize(vol), st.ownCost.data());
+    st.paths = dumps && dumps->paths ? dumps->paths
+                                     : (st.ownPaths.resize(vol * st.npaths), st.ownPaths.data());
+    st.sum = dumps && dumps->sum ? dumps->sum : (st.ownSum.resize(vol), st.ownSum.data());
+}
+
+int run(const oracle_params* p, const uint8_t* left, const uint8_t* right, int W, int H,
+        int16_t* out, const oracle_dumps* dumps, int threads) {
+    if (!valid_args(p, left, right, W, H) || !out) return 1;
+    Pipeline st;
+    setup(st, p, W, H, threads, dumps);
+    cost_stage(st, left, right);
+    path_stage(st, left);
+    finish(st, out, dumps);
+    return 0;
 }
 
 }  // namespace
@@ -411,41 +487,12 @@ extern "C" {
 /* StereoSGBM::compute (src/sgbm_gpu.cu:147-195) for grayscale input. */
 int oracle_sgbm(const oracle_params* p, const uint8_t* left, const uint8_t* right, int W, int H,
                 int16_t* out, const oracle_dumps* dumps) {
-    if (!valid_args(p, left, right, W, H) || !out) return 1;
-    Pipeline st;
-    setup(st, p, W, H);
-    cost_stage(st, left, right);
-    const size_t vol = static_cast<size_t>(W) * H * st.D;
-    for (int r = 0; r < st.npaths; ++r)
-        oracle_aggregate_path(st.cost.data(), left, W, H, st.D, p->P1, p->P2, st.cs, r,
-                              st.paths.data() + static_cast<size_t>(r) * vol);
-    finish(st, left, out, dumps);
-    return 0;
+    return run(p, left, right, W, H, out, dumps, 1);
 }
 
 int oracle_sgbm_mt(const oracle_params* p, const uint8_t* left, const uint8_t* right, int W,
-                   int H, int16_t* out, int nthreads) {
-    if (!valid_args(p, left, right, W, H) || !out) return 1;
-    if (nthreads < 1) nthreads = 1;
-    Pipeline st;
-    setup(st, p, W, H);
-    cost_stage(st, left, right);
-    const size_t vol = static_cast<size_t>(W) * H * st.D;
-    std::vector<std::thread> pool;
-    int next = 0;
-    while (next < st.npaths) {
-        pool.clear();
-        for (int t = 0; t < nthreads && next < st.npaths; ++t, ++next) {
-            const int r = next;
-            pool.emplace_back([&, r] {
-                oracle_aggregate_path(st.cost.data(), left, W, H, st.D, p->P1, p->P2, st.cs, r,
-                                      st.paths.data() + static_cast<size_t>(r) * vol);
-            });
-        }
-        for (auto& th : pool) th.join();
-    }
-    finish(st, left, out, nullptr);
-    return 0;
+                   int H, int16_t* out, int nthreads, const oracle_dumps* dumps) {
+    return run(p, left, right, W, H, out, dumps, nthreads);
 }
 
 }  // extern "C"

@@ -86,10 +86,12 @@ void oracle_median(const int16_t* in, int16_t* out, int W, int H);
 int oracle_sgbm(const oracle_params* p, const uint8_t* left, const uint8_t* right, int W, int H,
                 int16_t* out, const oracle_dumps* dumps);
 
-/* Same computation split over nthreads host threads (paths and rows in parallel); used only as
- * the reported CPU baseline. Output is identical to oracle_sgbm. */
+/* Same computation split over nthreads host threads: the cost, path sum and WTA in bands of
+ * rows, the paths one per thread. Every stage computes each element exactly as oracle_sgbm
+ * does, so outputs and dumps are identical to it. Used as the checker at full image sizes and
+ * as the reported CPU baseline. */
 int oracle_sgbm_mt(const oracle_params* p, const uint8_t* left, const uint8_t* right, int W,
-                   int H, int16_t* out, int nthreads);
+                   int H, int16_t* out, int nthreads, const oracle_dumps* dumps);
 
 #ifdef __cplusplus
 }

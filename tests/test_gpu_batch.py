@@ -124,3 +124,18 @@ def test_batch_rejects_invalid_pair(gpu_available):
     pairs.append((bad, pairs[0][1], pairs[0][2]))
     with pytest.raises(ls.StereoException, match="pair 2"):
         ls.BatchSGBM(arena_bytes=64 << 20).run(pairs)
+
+
+def test_batch_high_prefilter_caps(gpu_available):
+    """SAD with prefilter caps of 128 and more (the prefiltered byte wraps as the uint8 store
+    does) at block sizes 3, 5, 7 and 9, batched, equals the per-pair path and the oracle."""
+    pairs = []
+    for i, (cap, bs) in enumerate([(200, 3), (200, 5), (1000, 7), (1000, 5), (200, 9), (128, 5)]):
+        p = ls.StereoParams(maxDisparity=48, numDirections=4 + 4 * (i % 2), preFilterCap=cap,
+                            blockSize=bs, uniquenessRatio=0, speckleWindowSize=0)
+        l, r = ls.synth_pair(96 + 8 * i, 64, 2, 40, seed=500 + i)
+        pairs.append((p, l, r))
+    got = ls.BatchSGBM(arena_bytes=256 << 20).run(pairs)
+    for i, ((p, l, r), g) in enumerate(zip(pairs, got)):
+        np.testing.assert_array_equal(g, ls.StereoSGBM(p).compute(l, r), err_msg=f"pair {i}")
+        np.testing.assert_array_equal(g, oracle.sgbm(p, l, r), err_msg=f"pair {i} vs oracle")

@@ -35,6 +35,12 @@ CASES = [
     (96, 24, 0, 512, 0, 8, {}),           # D = 512, the largest range: 32 per lane
     (4600, 5, 0, 320, 1, 4, {}),          # the widest image: right-WTA row state fills LDS
     (4600, 5, 7, 263, 0, 8, {}),          # widest image, 16 per lane, minDisparity > 0
+    # Prefilter caps of 128 and more: g + cap exceeds 255 and wraps as the uint8 store does.
+    (96, 64, 0, 48, 0, 4, {"preFilterCap": 200, "blockSize": 3}),
+    (96, 64, 0, 48, 0, 8, {"preFilterCap": 200}),
+    (96, 64, 0, 48, 0, 4, {"preFilterCap": 1000, "blockSize": 7}),
+    (96, 64, 0, 48, 0, 4, {"preFilterCap": 1000, "blockSize": 5}),
+    (96, 64, 0, 48, 0, 4, {"preFilterCap": 200, "blockSize": 9}),
 ]
 
 

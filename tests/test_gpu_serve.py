@@ -151,3 +151,27 @@ def test_submit_and_wait_from_several_threads(gpu_available):
     assert st["completed"] == len(pairs) and st["failed"] == 0
     for i, (g, w) in enumerate(zip(got, want)):
         np.testing.assert_array_equal(g, w, err_msg=f"pair {i}")
+
+
+def test_wait_timeout_keeps_the_request(gpu_available):
+    """A wait that times out leaves the request queued or running with its buffers alive; a
+    later wait returns its result, equal to the per-pair path's. A zero timeout on a 1080p
+    request submitted behind three others times out (it cannot be done in no time)."""
+    p = ls.StereoParams(maxDisparity=128, numDirections=8,
+                        matchingCostMode=ls.MatchingCostMode.Census, uniquenessRatio=0,
+                        speckleWindowSize=0)
+    l, r = ls.synth_pair(1920, 1080, 4, 120, seed=31)
+    want = ls.StereoSGBM(p).compute(l, r)
+    srv = ls.ServeSGBM(pool_bytes=24 << 30, lanes=1, max_batch=1)
+    tickets = [srv.submit(p, l, r) for _ in range(4)]
+    timed_out = 0
+    for t in tickets:
+        try:
+            out, info = srv.wait(t, timeout_s=0)
+        except TimeoutError:
+            timed_out += 1
+            out, info = srv.wait(t)
+        assert info["status"] == 0
+        np.testing.assert_array_equal(out, want)
+    srv.close()
+    assert timed_out >= 1

@@ -321,6 +321,121 @@ def test_speckle_snapshot_known_answer():
     assert oracle.speckle(d, 2, 16)[2, 2] == -1
 
 
+def _speckle_padded(d, min_similar, max_diff):
+    """Speckle (src/sgbm_kernels.cu:979-1013) counted on a zero-padded copy: 25 shifted
+    views of the snapshot instead of a bounds-checked window walk. Zero is not > 0, so padding
+    adds nothing, which is what skipping out-of-image neighbours does."""
+    H, W = d.shape
+    pad = np.zeros((H + 4, W + 4), np.int32)
+    pad[2:-2, 2:-2] = d
+    c = d.astype(np.int32)
+    n = np.zeros((H, W), np.int32)
+    for dy in range(5):
+        for dx in range(5):
+            v = pad[dy:dy + H, dx:dx + W]
+            n += (v > 0) & (np.abs(c - v) <= max_diff)
+    out = d.copy()
+    out[(c > 0) & (n < min_similar)] = -1
+    return out
+
+
+def _speckle_in_place(d, min_similar, max_diff):
+    """The reference kernel run by one thread in raster order, updating in place (one of the
+    orders its racing threads can produce)."""
+    d = d.copy()
+    H, W = d.shape
+    for y in range(H):
+        for x in range(W):
+            c = int(d[y, x])
+            if c <= 0:
+                continue
+            n = sum(1 for j in range(max(0, y - 2), min(H, y + 3))
+                    for i in range(max(0, x - 2), min(W, x + 3))
+                    if d[j, i] > 0 and abs(c - int(d[j, i])) <= max_diff)
+            if n < min_similar:
+                d[y, x] = -1
+    return d
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_speckle_random_maps_equal_padded_count(seed):
+    r = np.random.default_rng(seed)
+    H, W = int(r.integers(1, 12)), int(r.integers(1, 14))
+    # Values around a few levels, holes and zeros, so counts vary across the threshold.
+    d = (r.integers(0, 4, (H, W)) * 40 + r.integers(-6, 7, (H, W))).astype(np.int16)
+    d[r.random((H, W)) < 0.2] = -1
+    d[r.random((H, W)) < 0.1] = 0
+    for min_similar in (1, 4, 9, 25):
+        for max_diff in (0, 5, 16):
+            np.testing.assert_array_equal(oracle.speckle(d, min_similar, max_diff),
+                                          _speckle_padded(d, min_similar, max_diff))
+
+
+def test_speckle_border_windows():
+    # A constant valid map: a pixel's count is the number of in-image pixels of its window,
+    # 9 at a corner, 12 next to it, 15 on an edge two pixels in, 25 inside.
+    d = np.full((7, 8), 48, np.int16)
+    counts = np.array([[min(y, 2) + min(6 - y, 2) + 1 for _ in range(8)] for y in range(7)]) * \
+        np.array([[min(x, 2) + min(7 - x, 2) + 1 for x in range(8)] for _ in range(7)])
+    assert counts[0, 0] == 9 and counts[0, 1] == 12 and counts[3, 3] == 25
+    for min_similar in (9, 10, 12, 13, 15, 16, 25):
+        want = np.where(counts >= min_similar, 48, -1).astype(np.int16)
+        np.testing.assert_array_equal(oracle.speckle(d, min_similar, 0), want)
+
+
+def test_speckle_snapshot_differs_from_in_place():
+    # One row of six equal values with minSimilar 4: the end pixels see 3 similar pixels, the
+    # others 4 or 5. Reading the snapshot, only the ends go. Updating in raster order, each
+    # removal drops the next pixel's count below 4 and the whole row goes.
+    d = np.full((1, 6), 100, np.int16)
+    snap = oracle.speckle(d, 4, 0)
+    np.testing.assert_array_equal(snap, [[-1, 100, 100, 100, 100, -1]])
+    np.testing.assert_array_equal(_speckle_in_place(d, 4, 0), np.full((1, 6), -1))
+
+
+def test_speckle_window_mapping_in_pipeline():
+    """compute maps speckleWindowSize to min(speckleWindowSize, 25) similar pixels and
+    speckleRange to 16*speckleRange (src/sgbm_gpu.cu:480-488): the pipeline's after-speckle map
+    equals the padded count applied to its after-LR map with those values."""
+    import lumenstereo_amd as ls
+    left, right = ls.synth_pair(64, 40, 2, 28, seed=77)
+    for sws, rng_ in ((100, 2), (25, 2), (24, 2), (7, 1), (3, 0)):
+        p = dict(minDisparity=0, maxDisparity=32, blockSize=5, P1=200, P2=800, preFilterCap=31,
+                 uniquenessRatio=0, disp12MaxDiff=1, speckleWindowSize=sws, speckleRange=rng_,
+                 useGPU=1, useSharedMemory=1, numDirections=4, matchingCostMode=0)
+        _, st = oracle.sgbm(p, left, right, dumps=True)
+        np.testing.assert_array_equal(st["after_speckle"],
+                                      _speckle_padded(st["after_lr"], min(sws, 25), 16 * rng_))
+    # 100 and 25 are the same filter; 24 keeps pixels that 25 removes.
+    outs = {}
+    for sws in (100, 25, 24):
+        p["speckleWindowSize"], p["speckleRange"] = sws, 2
+        outs[sws] = oracle.sgbm(p, left, right, dumps=True)[1]["after_speckle"]
+    np.testing.assert_array_equal(outs[100], outs[25])
+    assert np.count_nonzero(outs[24] != outs[25]) > 0
+
+
+@pytest.mark.parametrize("case", [(130, 77, 0, 40, 0, 8, {}), (96, 64, 3, 67, 1, 4, {}),
+                                  (100, 70, 0, 32, 0, 8, {"speckleWindowSize": 20}),
+                                  (90, 40, 0, 290, 1, 4, {"uniquenessRatio": 10})])
+def test_threaded_oracle_equals_single_threaded(case):
+    """The threaded form (bands of rows, one path per thread) that checks full-size frames
+    gives exactly the single-threaded oracle's output and stage dumps."""
+    import lumenstereo_amd as ls
+    W, H, minD, maxD, mode, paths, extra = case
+    left, right = ls.synth_pair(W, H, 2, maxD - minD - 4, seed=W + H)
+    p = dict(minDisparity=minD, maxDisparity=maxD, blockSize=5, P1=200, P2=800,
+             preFilterCap=31, uniquenessRatio=0, disp12MaxDiff=1, speckleWindowSize=0,
+             speckleRange=2, useGPU=1, useSharedMemory=1, numDirections=paths,
+             matchingCostMode=mode)
+    p.update(extra)
+    o1, s1 = oracle.sgbm(p, left, right, dumps=True)
+    o7, s7 = oracle.sgbm(p, left, right, dumps=True, threads=7)
+    np.testing.assert_array_equal(o1, o7)
+    for k in s1:
+        np.testing.assert_array_equal(s1[k], s7[k], err_msg=k)
+
+
 def test_full_pipeline_rejects_bad_ranges():
     p = dict(minDisparity=0, maxDisparity=0, blockSize=5, P1=200, P2=800, preFilterCap=31,
              uniquenessRatio=0, disp12MaxDiff=1, speckleWindowSize=0, speckleRange=32,
