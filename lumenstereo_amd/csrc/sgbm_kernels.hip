@@ -37,6 +37,15 @@ __global__ __launch_bounds__(256) void cost_sad_direct_kernel(PairJob j) {
     gmem(j.cost)[item] = sad_cost_direct(j, static_cast<int>(pix % j.W), static_cast<int>(pix / j.W), d);
 }
 
+// Every cost 255 (an empty SAD window). A kernel rather than a memset node, so the stage can
+// be captured into a graph.
+__global__ __launch_bounds__(256) void cost_fill_kernel(PairJob j) {
+    const size_t n16 = (size_t)j.W * j.H * j.Dp / 16;  // Dp is a multiple of 64
+    LUMEN_G u32x4* dst = reinterpret_cast<LUMEN_G u32x4*>(gmem(j.cost));
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += (size_t)gridDim.x * 256)
+        dst[i] = u32x4{~0u, ~0u, ~0u, ~0u};
+}
+
 hipError_t launch_cost(const PairJob& j, hipStream_t s) {
     if (j.mode == 1) {
         switch (j.sched.censusTile) {
@@ -49,7 +58,8 @@ hipError_t launch_cost(const PairJob& j, hipStream_t s) {
     const int r = j.blockSize / 2;
     if (r < 0) {
         // A negative radius gives an empty window: every cost is 255.
-        return hipMemsetAsync(j.cost, 0xFF, (size_t)j.W * j.H * j.Dp, s);
+        hipLaunchKernelGGL(cost_fill_kernel, dim3(1024), dim3(256), 0, s, j);
+        return hipGetLastError();
     }
     if (sad_lds_bytes(r, j.Dp) <= kSadMaxLds) {
         dim3 grid((j.W + kSadTile - 1) / kSadTile, j.H);

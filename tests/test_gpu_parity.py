@@ -41,6 +41,7 @@ CASES = [
     (96, 64, 0, 48, 0, 4, {"preFilterCap": 1000, "blockSize": 7}),
     (96, 64, 0, 48, 0, 4, {"preFilterCap": 1000, "blockSize": 5}),
     (96, 64, 0, 48, 0, 4, {"preFilterCap": 200, "blockSize": 9}),
+    (70, 50, 0, 16, 0, 4, {"blockSize": -3}),    # empty window: every cost 255 (fill kernel)
 ]
 
 
