@@ -89,6 +89,23 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
 int lumen_sgbm_compute_device(lumen_sgbm* h, const uint8_t* d_left, const uint8_t* d_right,
                               int width, int height, int16_t* d_out, int sync);
 
+/* Allocation-free form of lumen_sgbm_compute_device. It has no counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It takes the parameters instead of a handle
+ * and enqueues the same stages on `stream` (a hipStream_t; NULL = the null stream) without
+ * waiting; it calls no hipMalloc, hipFree or other allocating routine. The pair's working set
+ * (W*H*Dp*(1 + 2*paths) bytes, Dp = D rounded up to 64, plus five int16 maps) lives in a fixed
+ * 192 MiB device array of the library, one per GPU, held until the process ends: 640x480 D=64
+ * with 4 paths fits, 1920x1080 D=128 does not; a larger pair fails with LUMEN_EINVAL and
+ * lumen_last_error() gives the sizes. Every call on one GPU uses that GPU's array, so calls
+ * that overlap on one GPU (other threads or streams before this call's work has finished)
+ * race and give undefined results. No stage reads scratch it has not written in the same
+ * call, so nothing is cleared between calls. Validation and messages are those of
+ * lumen_sgbm_compute; the schedule is the tuned one. */
+int lumen_sgbm_compute_device_noalloc(const lumen_stereo_params* params, const uint8_t* d_left,
+                                      const uint8_t* d_right, int width, int height,
+                                      int16_t* d_out, void* stream);
+
 /* The hipStream_t the handle launches on (as void*), creating it if needed. */
 int lumen_sgbm_get_stream(lumen_sgbm* h, void** stream_out);
 

@@ -229,6 +229,8 @@ _SIGNATURES = {
                                     _c_int, _c_int, _c_void_p, _c_size_t]),
     "lumen_sgbm_compute_device": (_c_int, [_c_void_p, _c_void_p, _c_void_p, _c_int, _c_int,
                                            _c_void_p, _c_int]),
+    "lumen_sgbm_compute_device_noalloc": (_c_int, [ctypes.POINTER(CParams), _c_void_p, _c_void_p,
+                                                   _c_int, _c_int, _c_void_p, _c_void_p]),
     "lumen_sgbm_get_stream": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
     "lumen_sgbm_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "lumen_sgbm_get_timings_avg": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(ctypes.c_float)]),

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -268,6 +269,63 @@ int check_params(const lumen_stereo_params* p) {
 
 void lumen::set_last_error(const std::string& msg) { g_last_error = msg; }
 
+// ---------------------------------------------------------------------------------------
+// Scratch of lumen_sgbm_compute_device_noalloc: a fixed device array that holds one pair's
+// working set (cost volume, per-path costs, the intermediate maps, the aggregation's trash
+// rows), so the entry point never allocates. Each device the library runs on holds its own
+// copy for the life of the process. In host code `scratch` is only a handle for the symbol:
+// its device address comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kNoAllocScratchBytes = size_t(192) << 20;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch (looked up on the first call per device).
+hipError_t scratch_ptr(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
+// Points j's intermediate buffers into the scratch at `base` (256-byte aligned pieces, as
+// layout_pair does) and returns the bytes used; the trash rows are returned in `trash`.
+size_t layout_scratch(PairJob& j, char* base, uint16_t** trash) {
+    const size_t n = (size_t)j.W * j.H;
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        char* q = base ? base + used : nullptr;
+        used += align_up(bytes, 256);
+        return q;
+    };
+    j.cost = reinterpret_cast<uint8_t*>(take(j.pathStride));
+    j.L = reinterpret_cast<uint16_t*>(take(j.pathStride * 2 * j.npaths));
+    j.dispL = reinterpret_cast<int16_t*>(take(n * 2));
+    j.dispR = reinterpret_cast<int16_t*>(take(n * 2));
+    j.dispLR = reinterpret_cast<int16_t*>(take(n * 2));
+    j.wtaInt = reinterpret_cast<int16_t*>(take(n * 2));
+    j.spk = reinterpret_cast<int16_t*>(take(n * 2));
+    *trash = reinterpret_cast<uint16_t*>(take(kTrashElems * 2));
+    return used;
+}
+
+}  // namespace
+
 extern "C" {
 
 int lumen_sgbm_default_params(lumen_stereo_params* out) {
@@ -446,6 +504,62 @@ int lumen_sgbm_compute_device(lumen_sgbm* h, const uint8_t* d_left, const uint8_
         if (e != hipSuccess) return h->hip_fail(e, "compute");
     }
     h->haveResult = true;
+    return LUMEN_OK;
+}
+
+int lumen_sgbm_compute_device_noalloc(const lumen_stereo_params* params, const uint8_t* d_left,
+                                      const uint8_t* d_right, int width, int height,
+                                      int16_t* d_out, void* stream) {
+    if (check_params(params)) return LUMEN_EINVAL;
+    const std::string msg = validate_pair(*params, width, height, d_left && d_right);
+    if (!msg.empty()) {
+        g_last_error = msg;
+        return LUMEN_EINVAL;
+    }
+    if (!d_out) {
+        g_last_error = "Output buffer is NULL";
+        return LUMEN_EINVAL;
+    }
+    PairJob j{};
+    fill_job(j, *params, width, height);
+    uint16_t* trash = nullptr;
+    const size_t need = layout_scratch(j, nullptr, &trash);
+    if (need > kNoAllocScratchBytes) {
+        g_last_error = "pair needs " + std::to_string(need) +
+                       " bytes of working memory; lumen_sgbm_compute_device_noalloc has " +
+                       std::to_string(kNoAllocScratchBytes) +
+                       " (W*H*Dp*(1 + 2*paths) plus five int16 maps)";
+        return LUMEN_EINVAL;
+    }
+    void* base = nullptr;
+    hipError_t e = scratch_ptr(&base);
+    if (e != hipSuccess) {
+        g_last_error = std::string("scratch address: ") + hipGetErrorString(e);
+        return LUMEN_EDEVICE;
+    }
+    layout_scratch(j, static_cast<char*>(base), &trash);
+    j.left = d_left;
+    j.right = d_right;
+    j.out = d_out;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const char* what = "cost launch";
+    e = launch_cost(j, s);
+    if (e == hipSuccess) {
+        what = "path aggregation launch";
+        e = launch_paths(j, trash, s);
+    }
+    if (e == hipSuccess) {
+        what = "WTA launch";
+        e = launch_wta(j, s);
+    }
+    if (e == hipSuccess) {
+        what = "post-processing launch";
+        e = launch_post(j, s);
+    }
+    if (e != hipSuccess) {
+        g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+        return LUMEN_EDEVICE;
+    }
     return LUMEN_OK;
 }
 
