@@ -155,7 +155,12 @@ typedef struct lumen_schedule {
                               block's stores fit: padded range <= 192 or 256, 384) */
     int32_t agg_priority;  /* aggregation wave priority: 0 none, 1 by path kind, 2 by length */
     int32_t census_tile;   /* census cost segment width in pixels: 128, 256 or 512 */
-    int32_t reserved;
+    int32_t agg_wide;      /* aggregation paths whose scanlines span 32 lanes (two per wave,
+                              half the disparities per lane: a shorter dependency chain per
+                              step for twice the waves) instead of 16 (four per wave): 0 none,
+                              1 the straight pair along the longer image dimension, 2 all four
+                              straight paths, 3 every path. Non-zero only for padded ranges 64,
+                              128, 256 and 512. */
 } lumen_schedule;
 
 /* Overrides the tuned schedule for this handle (NULL restores it). Checked at compute time:

@@ -394,7 +394,7 @@ int lumen_sgbm_set_schedule(lumen_sgbm* h, const lumen_schedule* s) {
         h->hasSchedule = false;
         return LUMEN_OK;
     }
-    h->schedule = Schedule{s->agg_prefetch, s->agg_priority, s->census_tile};
+    h->schedule = Schedule{s->agg_prefetch, s->agg_priority, s->census_tile, s->agg_wide};
     h->hasSchedule = true;
     return LUMEN_OK;
 }
@@ -410,7 +410,7 @@ int lumen_sgbm_get_schedule(const lumen_sgbm* h, int width, int height, lumen_sc
     out->agg_prefetch = s.aggPrefetch;
     out->agg_priority = s.aggPriority;
     out->census_tile = s.censusTile;
-    out->reserved = 0;
+    out->agg_wide = s.aggWide;
     return LUMEN_OK;
 }
 

@@ -21,6 +21,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "sgbm_internal.h"
 
 namespace lumen {
@@ -51,11 +53,15 @@ constexpr int kRowRor1 = 0x121;        // lane i reads lane (i-1) mod 16 of its 
 constexpr int kRowRor15 = 0x12F;       // lane i reads lane (i+1) mod 16 of its row
 constexpr int kRowMirror = 0x140;      // lane i reads lane 15-i of its row
 constexpr int kRowHalfMirror = 0x141;  // lane i reads lane 7-i of its half row
+constexpr int kWaveShr1 = 0x138;       // lane i reads lane i-1 of the wave (lane 0: no source)
+constexpr int kWaveShl1 = 0x130;       // lane i reads lane i+1 of the wave (lane 63: no source)
 
-// v of the lane CTRL names. Only controls that give every lane a source are used (the ones
-// above and row_newbcast), so no lane keeps an old value and the bound_ctrl form is exact; it
-// also lets the compiler fold the move into the VALU instruction that consumes it (v_min_u32
-// with a DPP source), one instruction per butterfly stage instead of three.
+// v of the lane CTRL names. The row controls give every lane a source (the ones above and
+// row_newbcast), so no lane keeps an old value and the bound_ctrl form is exact; it also lets
+// the compiler fold the move into the VALU instruction that consumes it (v_min_u32 with a DPP
+// source), one instruction per butterfly stage instead of three. The wave shifts leave lane 0
+// (shr) or lane 63 (shl) without a source: those lanes read 0, and their users mask them (they
+// stand for disparity -1 or Dp).
 //
 // Contract: every caller keeps whole 16-lane rows active (no divergent branch around a DPP
 // use that disables part of a row). A DPP read of a disabled source lane returns 0 in the
@@ -595,23 +601,28 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
     step = dy * W + dx;
 }
 
-// Slice layout. Lane l of a DPP row holds K of a pixel's Dp = 16K disparities, in K/Q chunks of
-// Q consecutive ones (Q = 8 when K is a multiple of 8, else 4): chunk c is d = 16Qc + Ql ..
-// 16Qc + Ql + Q-1. A chunk is one 4- or 8-byte cost load and one 8- or 16-byte path-cost access
-// per lane, and the 16 lanes' chunks are contiguous, so every access instruction of a row
-// covers one contiguous 64 to 256 bytes whatever K is. For K <= 8 there is one chunk and lane l
-// simply holds d = lK .. lK+K-1. Registers hold the lane's elements chunk by chunk, element e
-// being disparity slice_d<K>(l, e).
-template <int K>
-__host__ __device__ constexpr int slice_q() { return K % 8 == 0 ? 8 : 4; }
-
-template <int K>
-__device__ __forceinline__ int slice_d(int l, int e) {
-    constexpr int Q = slice_q<K>();
-    return 16 * Q * (e / Q) + Q * l + e % Q;
+// Slice layout. A scanline's disparity vector is spread over LPS lanes (a 16-lane DPP row, or
+// 32 lanes: two rows), lane l holding K of the pixel's Dp = LPS*K disparities.
+//
+// LPS = 16: K/Q chunks of Q consecutive disparities (Q = 8 when K is a multiple of 8, else 4):
+// chunk c of lane l is d = 16Qc + Ql .. 16Qc + Ql + Q-1. A chunk is one 4- or 8-byte cost load
+// and one 8- or 16-byte path-cost access per lane, and the 16 lanes' chunks are contiguous, so
+// every access instruction of a row covers one contiguous 64 to 256 bytes whatever K is. For
+// K <= 8 there is one chunk and lane l simply holds d = lK .. lK+K-1.
+// LPS = 32 (K = 2, 4, 8 or 16): one chunk, lane l holds d = lK .. lK+K-1.
+// Registers hold the lane's elements chunk by chunk, element e being slice_d<K, LPS>(l, e).
+template <int K, int LPS = 16>
+__host__ __device__ constexpr int slice_q() {
+    return LPS == 32 ? K : (K % 8 == 0 ? 8 : 4);
 }
 
-// Lane that holds disparity d, and its element index there.
+template <int K, int LPS = 16>
+__device__ __forceinline__ int slice_d(int l, int e) {
+    constexpr int Q = slice_q<K, LPS>();
+    return LPS * Q * (e / Q) + Q * l + e % Q;
+}
+
+// Lane that holds disparity d, and its element index there (16-lane rows).
 template <int K>
 __device__ __forceinline__ int slice_lane(int d) {
     constexpr int Q = slice_q<K>();
@@ -623,43 +634,60 @@ __device__ __forceinline__ int slice_elem(int d) {
     return d / (16 * Q) * Q + d % Q;
 }
 
+// A lane's K cost bytes, four to a word (for K = 2 the low half of one word).
 template <int K>
 struct CostBytes {
-    uint32_t w[K / 4];
+    uint32_t w[(K + 3) / 4];
 };
 
 // p: the pixel's cost vector + Q*l (the lane's first chunk).
-template <int K>
+template <int K, int LPS = 16>
 __device__ __forceinline__ CostBytes<K> load_cost(const LUMEN_G uint8_t* p) {
-    constexpr int Q = slice_q<K>();
+    constexpr int Q = slice_q<K, LPS>();
     CostBytes<K> c;
 #pragma unroll
     for (int ch = 0; ch < K / Q; ++ch) {
-        if constexpr (Q == 8) {
-            const u32x2 v = *reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch);
+        const LUMEN_G uint8_t* q = p + LPS * Q * ch;
+        if constexpr (Q == 2) {
+            c.w[0] = *reinterpret_cast<const LUMEN_G uint16_t*>(q);
+        } else if constexpr (Q == 4) {
+            c.w[ch] = *reinterpret_cast<const LUMEN_G uint32_t*>(q);
+        } else if constexpr (Q == 8) {
+            const u32x2 v = *reinterpret_cast<const LUMEN_G u32x2*>(q);
             c.w[2 * ch + 0] = v.x;
             c.w[2 * ch + 1] = v.y;
         } else {
-            c.w[ch] = *reinterpret_cast<const LUMEN_G uint32_t*>(p + 16 * Q * ch);
+            static_assert(Q == 16, "chunk of 2, 4, 8 or 16 disparities");
+            const u32x4 v = *reinterpret_cast<const LUMEN_G u32x4*>(q);
+            c.w[4 * ch + 0] = v.x;
+            c.w[4 * ch + 1] = v.y;
+            c.w[4 * ch + 2] = v.z;
+            c.w[4 * ch + 3] = v.w;
         }
     }
     return c;
 }
 
 // p: the pixel's path-cost vector + Q*l. Each chunk store writes a whole contiguous run of
-// the row's 16 lanes, so the stores are nontemporal.
-template <int K>
+// the scanline's lanes, so the stores are nontemporal.
+template <int K, int LPS = 16>
 __device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)[K / 2]) {
-    constexpr int Q = slice_q<K>();
+    constexpr int Q = slice_q<K, LPS>();
 #pragma unroll
     for (int ch = 0; ch < K / Q; ++ch) {
-        if constexpr (Q == 8) {
-            __builtin_nontemporal_store(u32x4{bits(v[4 * ch]), bits(v[4 * ch + 1]),
-                                              bits(v[4 * ch + 2]), bits(v[4 * ch + 3])},
-                                        reinterpret_cast<LUMEN_G u32x4*>(p + 16 * Q * ch));
+        LUMEN_G uint16_t* q = p + LPS * Q * ch;
+        const int b = ch * (Q / 2);
+        if constexpr (Q == 2) {
+            __builtin_nontemporal_store(bits(v[b]), reinterpret_cast<LUMEN_G uint32_t*>(q));
+        } else if constexpr (Q == 4) {
+            __builtin_nontemporal_store(u32x2{bits(v[b]), bits(v[b + 1])},
+                                        reinterpret_cast<LUMEN_G u32x2*>(q));
         } else {
-            __builtin_nontemporal_store(u32x2{bits(v[2 * ch]), bits(v[2 * ch + 1])},
-                                        reinterpret_cast<LUMEN_G u32x2*>(p + 16 * Q * ch));
+#pragma unroll
+            for (int h = 0; h < Q / 8; ++h)
+                __builtin_nontemporal_store(u32x4{bits(v[b + 4 * h]), bits(v[b + 4 * h + 1]),
+                                                  bits(v[b + 4 * h + 2]), bits(v[b + 4 * h + 3])},
+                                            reinterpret_cast<LUMEN_G u32x4*>(q + 8 * h));
         }
     }
 }
@@ -667,33 +695,44 @@ __device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)
 // uint8 costs -> packed t16(C * costScale).
 template <int K>
 __device__ __forceinline__ void expand_cost(const CostBytes<K>& c, u16x2 cs, u16x2 (&out)[K / 2]) {
+    if constexpr (K == 2) {
+        out[0] = as16(__builtin_amdgcn_perm(0u, c.w[0], 0x0c010c00u)) * cs;
+    } else {
 #pragma unroll
-    for (int i = 0; i < K / 4; ++i) {
-        out[2 * i] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c010c00u)) * cs;
-        out[2 * i + 1] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c030c02u)) * cs;
+        for (int i = 0; i < K / 4; ++i) {
+            out[2 * i] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c010c00u)) * cs;
+            out[2 * i + 1] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c030c02u)) * cs;
+        }
     }
 }
 
-// d-1 and d+1 neighbour vectors of a packed slice (lane l of its DPP row). Inside a chunk
-// they are the lane's own neighbouring elements. At a chunk's start, d-1 is the last element
-// of the same chunk of lane l-1, or for lane 0 the last element of the previous chunk of lane
-// 15; at a chunk's end, d+1 is the first element of the same chunk of lane l+1, or for lane 15
-// of the next chunk of lane 0: one row rotation per chunk each way, then a lane select. The
-// values lane 0 gets for d = -1 and lane 15 for d = Dp are masked by the caller.
-template <int K>
+// d-1 and d+1 neighbour vectors of a packed slice (lane l of its scanline's lanes). Inside a
+// chunk they are the lane's own neighbouring elements. 16-lane rows: at a chunk's start, d-1 is
+// the last element of the same chunk of lane l-1, or for lane 0 the last element of the
+// previous chunk of lane 15; at a chunk's end, d+1 is the first element of the same chunk of
+// lane l+1, or for lane 15 of the next chunk of lane 0: one row rotation per chunk each way,
+// then a lane select. 32-lane scanlines (one chunk): the neighbours come from lanes l-1 and l+1
+// through whole-wave shifts, which cross the boundary between the two DPP rows. The values lane
+// 0 gets for d = -1 and the last lane for d = Dp are masked by the caller.
+template <int K, int LPS = 16>
 __device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&dm1)[K / 2],
                                            uint32_t (&dp1)[K / 2], int l) {
-    constexpr int Q = slice_q<K>(), NC = K / Q, PQ = Q / 2;
+    constexpr int Q = slice_q<K, LPS>(), NC = K / Q, PQ = Q / 2;
     uint32_t prevLast[NC], nextFirst[NC];
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]));
-        nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]));
+        if constexpr (LPS == 32) {
+            prevLast[c] = dpp<kWaveShr1>(bits(v[c * PQ + PQ - 1]));
+            nextFirst[c] = dpp<kWaveShl1>(bits(v[c * PQ]));
+        } else {
+            prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]));
+            nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]));
+        }
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         const uint32_t fromPrev = (c > 0 && l == 0) ? prevLast[c - 1] : prevLast[c];
-        const uint32_t fromNext = (c < NC - 1 && l == 15) ? nextFirst[c + 1] : nextFirst[c];
+        const uint32_t fromNext = (c < NC - 1 && l == LPS - 1) ? nextFirst[c + 1] : nextFirst[c];
 #pragma unroll
         for (int jj = 0; jj < PQ; ++jj) {
             const int q = c * PQ + jj;
@@ -713,6 +752,19 @@ __device__ __forceinline__ uint32_t slice_min(const u16x2 (&v)[K / 2]) {
     for (int j = 1; j < K / 2; ++j) mm = vmin(mm, v[j]);
     const uint32_t b = bits(mm);
     return min(b & 0xFFFFu, b >> 16);
+}
+
+// Minimum over the LPS lanes of a scanline, left in each of them: the DPP row minimum, and
+// for 32 lanes the minimum with the other row of the pair (v_permlane16_swap exchanges rows 0
+// and 1, 2 and 3).
+template <int LPS>
+__device__ __forceinline__ uint32_t group_min(uint32_t v) {
+    v = row_min(v);
+    if constexpr (LPS == 32) {
+        const auto sw = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = min(static_cast<uint32_t>(sw[0]), static_cast<uint32_t>(sw[1]));
+    }
+    return v;
 }
 
 // The 256-entry t16(P2adj) table (src/sgbm_kernels.cu:304-310) lives in registers: lane l
@@ -741,24 +793,31 @@ __device__ __forceinline__ uint32_t p2_lookup(const P2Table& t, uint32_t g) {
     return (g & 1u) ? (w >> 16) : (w & 0xFFFFu);
 }
 
-// Scanlines s0 .. s0+3 of `path` of pair j; wave-wide (no block synchronisation). Lanes of a
-// DPP row whose scanline does not exist compute on a copy of pixel 0 and store to `trash`.
-// PF is the prefetch depth (a schedule choice; the results do not depend on it).
-template <int K, bool PAD, int PF = default_prefetch(K)>
+// Scanlines s0 .. s0 + 64/LPS - 1 of `path` of pair j (one per LPS lanes); wave-wide (no block
+// synchronisation). Lanes whose scanline does not exist compute on a copy of pixel 0 and store
+// to `trash`. PF is the prefetch depth and LPS the lanes per scanline (schedule choices; the
+// results do not depend on them).
+template <int K, bool PAD, int PF = default_prefetch(K), int LPS = 16>
 __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_base) {
-    static_assert(valid_prefetch(K, PF), "unsupported prefetch depth for this slice");
+    static_assert(LPS == 16 || LPS == 32, "a scanline spans one or two DPP rows");
+    static_assert(LPS == 16 || K == 2 || K == 4 || K == 8 || K == 16,
+                  "32-lane scanlines hold 2, 4, 8 or 16 disparities per lane");
+    static_assert(valid_prefetch(K, PF, LPS), "unsupported prefetch depth for this slice");
     const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, l = lane & 15;
+    const int g = lane / LPS, l = lane & (LPS - 1);
     const int s = s0 + g;
     int p0, step, len;
     scan_geom(path, s, j.W, j.H, p0, step, len);
     int maxLen = max(len, __shfl_xor(len, 16));
     maxLen = max(maxLen, __shfl_xor(maxLen, 32));
     maxLen = __builtin_amdgcn_readfirstlane(maxLen);
+    int minLen = min(len, __shfl_xor(len, 16));
+    minLen = min(minLen, __shfl_xor(minLen, 32));
+    minLen = __builtin_amdgcn_readfirstlane(minLen);
     const P2Table p2t = make_p2(j.P1, j.P2, lane);
 
     const int Dp = j.Dp, D = j.D;
-    constexpr int Q = slice_q<K>();
+    constexpr int Q = slice_q<K, LPS>();
     const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
     LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
     const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
@@ -767,7 +826,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     uint32_t c2m[K / 2], padm[K / 2];
 #pragma unroll
     for (int q = 0; q < K / 2; ++q) {
-        const int dlo = slice_d<K>(l, 2 * q);
+        const int dlo = slice_d<K, LPS>(l, 2 * q);
         c2m[q] = (dlo == D - 1 ? 0xFFFFu : 0u) | (dlo + 1 == D - 1 ? 0xFFFF0000u : 0u);
         padm[q] = (dlo >= D ? 0xFFFFu : 0u) | (dlo + 1 >= D ? 0xFFFF0000u : 0u);
     }
@@ -777,53 +836,66 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     const LUMEN_G uint8_t* lp = cbase + (size_t)p0 * Dp;  // cost pointer of the last issued load
     int lt = 0;                                     // step index of the last issued load
     LUMEN_G uint16_t* sp = Lbase + (size_t)p0 * Dp;         // path-cost pointer of the step
-    // Stores of finished rows land in the trash: one Dp-sized vector per DPP row.
-    LUMEN_G uint16_t* const trash = gmem(trash_base) + g * 16 * K + l * Q;
+    // Stores of finished scanlines land in the trash: one Dp-sized vector per scanline.
+    LUMEN_G uint16_t* const trash = gmem(trash_base) + g * LPS * K + l * Q;
     u16x2 Lv[K / 2];
     {
-        const CostBytes<K> c0 = load_cost<K>(lp);
+        const CostBytes<K> c0 = load_cost<K, LPS>(lp);
         expand_cost<K>(c0, cs, Lv);
         if constexpr (PAD) {
 #pragma unroll
             for (int q = 0; q < K / 2; ++q) Lv[q] = as16(bits(Lv[q]) | padm[q]);
         }
-        store_path<K>(len > 0 ? sp : trash, Lv);
+        store_path<K, LPS>(len > 0 ? sp : trash, Lv);
     }
-    uint32_t M = row_min(slice_min<K>(Lv));
+    uint32_t M = group_min<LPS>(slice_min<K>(Lv));
 
     // Two register blocks of PF steps alternate: while one block is computed, the costs of the
-    // next are in flight. Lane l of a row also fetches the two image bytes of step
+    // next are in flight. Lane l of a scanline also fetches the two image bytes of step
     // tb + (l % PF) and looks up t16(P2adj) once per block; each step broadcasts its lane's
-    // value with DPP row_newbcast. Loads past the end of a scanline re-read its last pixel;
-    // steps past the end are computed but stored to the trash slot, so every store is
-    // unconditional and the wait counts stay exact.
+    // value with DPP row_newbcast (PF <= 16, so both rows of a 32-lane scanline hold the same
+    // values). CHECKED blocks handle scanline ends: loads past the end re-read the last pixel,
+    // and steps past the end are computed but stored to the trash slot, so every store is
+    // unconditional and the wait counts stay exact. While every scanline of the wave has all of
+    // a block's loads and stores inside it, the blocks skip those clamps (no per-step compares
+    // or 64-bit selects).
     const int last = len > 0 ? len - 1 : 0;
     const int lu = l & (PF - 1);
-    auto fetch = [&](int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
-        const int ti = min(tb + lu, last);
-        iv[0] = gmem(j.left)[p0 + ti * step];
-        iv[1] = gmem(j.left)[p0 + max(ti - 1, 0) * step];
+    auto fetch = [&](auto checked, int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
+        if constexpr (decltype(checked)::value) {
+            const int ti = min(tb + lu, last);
+            iv[0] = gmem(j.left)[p0 + ti * step];
+            iv[1] = gmem(j.left)[p0 + max(ti - 1, 0) * step];
 #pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            const bool adv = lt < last;
-            lp = adv ? lp + cstep : lp;
-            lt += adv ? 1 : 0;
-            dst[u] = load_cost<K>(lp);
+            for (int u = 0; u < PF; ++u) {
+                const bool adv = lt < last;
+                lp = adv ? lp + cstep : lp;
+                lt += adv ? 1 : 0;
+                dst[u] = load_cost<K, LPS>(lp);
+            }
+        } else {
+            const int ti = tb + lu;  // >= 1 and <= last
+            iv[0] = gmem(j.left)[p0 + ti * step];
+            iv[1] = gmem(j.left)[p0 + (ti - 1) * step];
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                lp += cstep;
+                dst[u] = load_cost<K, LPS>(lp);
+            }
+            lt += PF;
         }
     };
-    auto run = [&](int tb, const CostBytes<K>(&blk)[PF], const int (&iv)[2]) {
+    auto run = [&](auto checked, int tb, const CostBytes<K>(&blk)[PF], const int (&iv)[2]) {
         const uint32_t p2v = p2_lookup(p2t, static_cast<uint32_t>(abs(iv[0] - iv[1])));
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int t = tb + u;
-            const bool act = t < len;
             sp += step * Dp;
             const u16x2 c3 = splat(M + row_bcast(p2v, u));
             const u16x2 Mp = splat(M);
             u16x2 Ch[K / 2];
             expand_cost<K>(blk[u], cs, Ch);
             uint32_t dm1[K / 2], dp1[K / 2];
-            neighbours<K>(Lv, dm1, dp1, l);
+            neighbours<K, LPS>(Lv, dm1, dp1, l);
 #pragma unroll
             for (int q = 0; q < K / 2; ++q) {
                 uint32_t c1 = bits(as16(dm1[q]) + P1p);
@@ -839,29 +911,45 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
                 if constexpr (PAD) nv = as16(bits(nv) | padm[q]);
                 Lv[q] = nv;
             }
-            store_path<K>(act ? sp : trash, Lv);
-            M = row_min(slice_min<K>(Lv));
+            if constexpr (decltype(checked)::value)
+                store_path<K, LPS>(tb + u < len ? sp : trash, Lv);
+            else
+                store_path<K, LPS>(sp, Lv);
+            M = group_min<LPS>(slice_min<K>(Lv));
         }
     };
+    using Checked = std::integral_constant<bool, true>;
+    using Unchecked = std::integral_constant<bool, false>;
     CostBytes<K> bufA[PF], bufB[PF];
     int gA[2], gB[2];
-    fetch(1, bufA, gA);
+    fetch(Checked{}, 1, bufA, gA);
     // Entering the loop with block A complete keeps the loop-header state identical on both
     // edges, so no full drain is needed there.
     LUMEN_WAIT_VMCNT(0);
-    for (int t0 = 1; t0 < maxLen; t0 += 2 * PF) {
+    // Block A's loads were issued before block B's stores: waiting until only those stores may
+    // be outstanding retires the loads without waiting on the stores.
+    constexpr int N = PF * path_stores(K, LPS);
+    static_assert(N < 64, "vmcnt is 6 bits");
+    int t0 = 1;
+    // An iteration loads steps up to t0 + 3PF - 1 and stores steps up to t0 + 2PF - 1.
+    for (; t0 + 3 * PF <= minLen; t0 += 2 * PF) {
         // The scheduling barriers keep each block's loads where they are issued: moved next to
         // their first use they would stall there.
-        fetch(t0 + PF, bufB, gB);
+        fetch(Unchecked{}, t0 + PF, bufB, gB);
         __builtin_amdgcn_sched_barrier(0);
-        run(t0, bufA, gA);
-        fetch(t0 + 2 * PF, bufA, gA);
+        run(Unchecked{}, t0, bufA, gA);
+        fetch(Unchecked{}, t0 + 2 * PF, bufA, gA);
         __builtin_amdgcn_sched_barrier(0);
-        run(t0 + PF, bufB, gB);
-        // Block A's loads were issued before block B's stores: waiting until only those
-        // stores may be outstanding retires the loads without waiting on the stores.
-        constexpr int N = PF * path_stores(K);
-        static_assert(N < 64, "vmcnt is 6 bits");
+        run(Unchecked{}, t0 + PF, bufB, gB);
+        LUMEN_WAIT_VMCNT(N);
+    }
+    for (; t0 < maxLen; t0 += 2 * PF) {
+        fetch(Checked{}, t0 + PF, bufB, gB);
+        __builtin_amdgcn_sched_barrier(0);
+        run(Checked{}, t0, bufA, gA);
+        fetch(Checked{}, t0 + 2 * PF, bufA, gA);
+        __builtin_amdgcn_sched_barrier(0);
+        run(Checked{}, t0 + PF, bufB, gB);
         LUMEN_WAIT_VMCNT(N);
     }
 }

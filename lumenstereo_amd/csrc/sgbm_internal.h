@@ -19,14 +19,24 @@ constexpr int kTrashElems = 64 * 32;
 // Disparity-innermost layouts are padded to a multiple of 64 entries (16 lanes x 4).
 inline int padded_range(int D) { return (D + 63) / 64 * 64; }
 
-// Vector stores per aggregation step and lane: K/8 16-byte stores, or K/4 8-byte ones.
-__host__ __device__ constexpr int path_stores(int K) { return K % 8 == 0 ? K / 8 : K / 4; }
+// Vector stores per aggregation step and lane. 16-lane scanlines: K/8 16-byte stores, or K/4
+// 8-byte ones. 32-lane scanlines (K = 2, 4, 8, 16 consecutive disparities per lane): one 4-,
+// 8- or 16-byte store, or two 16-byte ones for K = 16.
+__host__ __device__ constexpr int path_stores(int K, int LPS = 16) {
+    return LPS == 32 ? (K <= 8 ? 1 : K / 8) : (K % 8 == 0 ? K / 8 : K / 4);
+}
 
 // Prefetch depths the aggregation supports for slice K: each lane of a DPP row fetches the
 // image bytes of at most one step of a block (PF <= 16), and the stores of one block must fit
 // the 6-bit vmcnt the loop waits on (PF * stores < 64).
-__host__ __device__ constexpr bool valid_prefetch(int K, int PF) {
-    return (PF == 4 || PF == 8 || PF == 16) && PF * path_stores(K) < 64;
+__host__ __device__ constexpr bool valid_prefetch(int K, int PF, int LPS = 16) {
+    return (PF == 4 || PF == 8 || PF == 16) && PF * path_stores(K, LPS) < 64;
+}
+
+// Lanes per scanline the aggregation supports for a padded range Dp: 16 always; 32 when each
+// lane then holds 2, 4, 8 or 16 disparities (Dp = 64, 128, 256, 512).
+__host__ __device__ constexpr bool wide_scanlines_ok(int Dp) {
+    return Dp == 64 || Dp == 128 || Dp == 256 || Dp == 512;
 }
 
 __host__ __device__ constexpr int default_prefetch(int K) { return K <= 8 ? 8 : 4; }
@@ -37,9 +47,12 @@ struct Schedule {
     int aggPrefetch;  // aggregation cost prefetch depth: 4, 8 or 16 (valid_prefetch)
     int aggPriority;  // aggregation wave priority: 0 none, 1 by path kind, 2 by scanline length
     int censusTile;   // census cost segment width: 128, 256 or 512 pixels
+    int aggWide;      // paths on 32-lane scanlines (two per wave, wide_scanlines_ok ranges):
+                      // 0 none, 1 the straight pair along the longer image dimension,
+                      // 2 all four straight paths, 3 every path; the others run 16-lane ones
 };
 
-inline Schedule default_schedule(int K) { return Schedule{default_prefetch(K), 1, 256}; }
+inline Schedule default_schedule(int K) { return Schedule{default_prefetch(K), 1, 256, 0}; }
 
 // Everything the device stages need to know about one stereo pair.
 struct PairJob {

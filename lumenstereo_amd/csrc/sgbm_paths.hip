@@ -8,10 +8,13 @@ namespace lumen {
 
 int scanline_count(int path, int W, int H) { return scan_count(path, W, H); }
 
-// Wave w takes scanlines 4w'..4w'+3 of the path whose range of waves contains w (paths in
-// order, so the long horizontal scanlines start first).
+// Wave w takes the next scanlines of the path whose range of waves contains w (paths in order,
+// so the long horizontal scanlines start first): four of 16 lanes each, or, for the paths in
+// wideMask, two of 32 lanes each (half the disparities per lane, so about a third fewer
+// instructions per step on each scanline's dependency chain, for twice the waves).
 struct PathsGrid {
     int waveStart[kMaxPaths + 1];
+    int wideMask;  // bit r: path r runs 32-lane scanlines
 };
 
 template <int K, bool PAD, int PF>
@@ -22,7 +25,8 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, 
 #pragma unroll
     for (int r = 1; r < kMaxPaths; ++r)
         if (r < j.npaths && wave >= g.waveStart[r]) path = r;
-    const int s0 = (wave - g.waveStart[path]) * 4;
+    const bool wide = (g.wideMask >> path) & 1;
+    const int s0 = (wave - g.waveStart[path]) * (wide ? 2 : 4);
     // The launch ends when its longest scanlines end. Priority 1: the longer straight path
     // first, then the other straight path, diagonals last; priority 2: by scanline length.
     if (j.sched.aggPriority == 1) {
@@ -35,6 +39,12 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, 
             __builtin_amdgcn_s_setprio(1);
     } else if (j.sched.aggPriority == 2) {
         set_priority_for_length(scan_length(path, s0, j.W, j.H));
+    }
+    if constexpr (K == 4 || K == 8 || K == 16 || K == 32) {
+        if (wide) {
+            paths_wave<K / 2, PAD, PF, 32>(j, path, s0, trash);
+            return;
+        }
     }
     paths_wave<K, PAD, PF>(j, path, s0, trash);
 }
@@ -66,12 +76,25 @@ static hipError_t launch_k(const PairJob& j, const PathsGrid& g, int blocks, uin
     }
 }
 
+// Paths that run 32-lane scanlines under aggregation lane setting `wide` (Schedule::aggWide).
+int wide_path_mask(int wide, int W, int H, int npaths) {
+    switch (wide) {
+        case 1: return W >= H ? 0x3 : 0xC;  // the straight pair along the longer dimension
+        case 2: return 0xF;                 // all four straight paths
+        case 3: return (1 << npaths) - 1;   // every path
+        default: return 0;
+    }
+}
+
 hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s) {
     PathsGrid g;
+    g.wideMask = wide_scanlines_ok(j.Dp) ? wide_path_mask(j.sched.aggWide, j.W, j.H, j.npaths) : 0;
+    if (j.sched.aggWide != 0 && !wide_scanlines_ok(j.Dp)) return hipErrorInvalidValue;
     int waves = 0;
     for (int r = 0; r < j.npaths; ++r) {
         g.waveStart[r] = waves;
-        waves += (scan_count(r, j.W, j.H) + 3) / 4;
+        const int per = (g.wideMask >> r) & 1 ? 2 : 4;
+        waves += (scan_count(r, j.W, j.H) + per - 1) / per;
     }
     for (int r = j.npaths; r <= kMaxPaths; ++r) g.waveStart[r] = waves;
     const int blocks = (waves + 3) / 4;

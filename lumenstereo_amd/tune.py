@@ -6,11 +6,16 @@ and ordered on the device, never what is computed:
 * ``agg_prefetch``: how many scanline steps of cost vectors the aggregation keeps in flight per
   wave (4, 8 or 16, as the slice allows);
 * ``agg_priority``: the wave issue priority within a SIMD (none, by path kind, by length);
-* ``census_tile``: the width of the census cost segment staged in LDS (128, 256 or 512 px).
+* ``census_tile``: the width of the census cost segment staged in LDS (128, 256 or 512 px);
+* ``agg_wide``: which aggregation paths run scanlines of 32 lanes (two per wave, half the
+  disparities per lane: fewer instructions on each step's dependency chain, twice the waves)
+  instead of 16 (four per wave): none, the straight pair along the longer image dimension, all
+  straight paths, or every path (padded ranges 64, 128, 256 and 512 only).
 
 For each tuning key (width bucket, padded disparity range Dp, cost mode, path count) the tuner
-times the candidates on the GPU. It is coordinate descent: first prefetch and priority on the
-aggregation stage time, then the census tile on the cost stage time. The candidates of a step
+times the candidates on the GPU. It is coordinate descent: first the lanes per scanline, then
+prefetch and priority, on the aggregation stage time; then the census tile on the cost stage
+time. The candidates of a step
 are timed round-robin over several rounds and each is scored by its median round, so a slow
 drift of the device's speed (which is as large as the differences being measured, a few per
 cent) falls on every candidate alike instead of favouring whichever was timed last. It checks
@@ -40,12 +45,19 @@ def path_stores(K: int) -> int:
 
 
 def valid_prefetch(K: int, pf: int) -> bool:
-    """Mirror of valid_prefetch in sgbm_internal.h."""
+    """Mirror of valid_prefetch in sgbm_internal.h (16-lane scanlines; the 32-lane ones store
+    at most two vectors per step, so every depth fits them)."""
     return pf in (4, 8, 16) and pf * path_stores(K) < 64
 
 
+def wide_settings(K: int):
+    """agg_wide values a slice of K disparities per 16 lanes supports (wide_scanlines_ok)."""
+    return (0, 1, 2, 3) if 16 * K in (64, 128, 256, 512) else (0,)
+
+
 def default_schedule(K: int) -> dict:
-    return {"agg_prefetch": 8 if K <= 8 else 4, "agg_priority": 1, "census_tile": 256}
+    return {"agg_prefetch": 8 if K <= 8 else 4, "agg_priority": 1, "census_tile": 256,
+            "agg_wide": 0}
 
 
 def agg_candidates(K: int):
@@ -61,7 +73,7 @@ def tuning_keys(quick: bool = False):
     keys = []
     widths = (640, 1920) if quick else (320, 640, 1280, 1920)
     for W in widths:
-        for Dp in (64, 128, 192):
+        for Dp in (64, 128, 192, 256):
             for census in (False, True):
                 for paths in (4, 8):
                     keys.append((W, WIDTHS[W], Dp, Dp, census, paths))
@@ -131,14 +143,19 @@ def tune(keys, reps_small: int = 10, rounds: int = 5, log=print):
         reps = reps_small if W * H * Dp < 400e6 else 4
         m = StereoSGBM(p)
         base = default_schedule(K)
-        cands = [dict(base)] + [dict(base, agg_prefetch=pf, agg_priority=prio)
-                                for pf, prio in agg_candidates(K)
-                                if (pf, prio) != (base["agg_prefetch"], base["agg_priority"])]
+        wcands = [dict(base, agg_wide=w) for w in wide_settings(K)]
+        wtimes, wouts = timed_rounds(m, d_in, (H, W), reps, wcands, 2,
+                                     rounds if len(wcands) > 1 else 1)
+        ref = wouts[0]
+        mismatches = sum(int(not np.array_equal(o, ref)) for o in wouts)
+        wb = wcands[pick(wtimes, 0)]
+        cands = [dict(wb)] + [dict(wb, agg_prefetch=pf, agg_priority=prio)
+                              for pf, prio in agg_candidates(K)
+                              if (pf, prio) != (base["agg_prefetch"], base["agg_priority"])]
         times, outs = timed_rounds(m, d_in, (H, W), reps, cands, 2, rounds)
-        ref = outs[0]
-        mismatches = sum(int(not np.array_equal(o, ref)) for o in outs)
+        mismatches += sum(int(not np.array_equal(o, ref)) for o in outs)
         b = pick(times, 0)
-        best, t_def_agg, best_t = cands[b], times[0], times[b]
+        best, t_def_agg, best_t = cands[b], wtimes[0], times[b]
         tiles = (128, 256, 512) if census else (base["census_tile"],)
         tcands = [dict(best, census_tile=t) for t in tiles]
         ctimes, couts = timed_rounds(m, d_in, (H, W), reps, tcands, 1, rounds if census else 1)
@@ -150,7 +167,7 @@ def tune(keys, reps_small: int = 10, rounds: int = 5, log=print):
             raise RuntimeError(f"{mismatches} schedules changed the output of key {W} {Dp} "
                                f"{int(census)} {paths}")
         entries.append((W, Dp, int(census), paths, best["agg_prefetch"], best["agg_priority"],
-                        best["census_tile"]))
+                        best["census_tile"], best["agg_wide"]))
         report.append((W, H, Dp, D, census, paths, t_def_agg, best_t, t_def_cost, best_cost, best))
         log(f"# {W}x{H} Dp={Dp} {'census' if census else 'sad'} {paths}p: agg {t_def_agg:.3f} -> "
             f"{best_t:.3f} ms, cost {t_def_cost:.3f} -> {best_cost:.3f} ms, {best}")
@@ -163,7 +180,7 @@ def write_table(entries, path: Path, command: str) -> None:
     lines = [
         "# Tuned per-pair schedules (written by lumenstereo_amd/tune.py; read by liblumen_sgbm.so).",
         f"# {datetime.date.today().isoformat()}: {command}",
-        "# width_max Dp mode(1=census) paths agg_prefetch agg_priority census_tile",
+        "# width_max Dp mode(1=census) paths agg_prefetch agg_priority census_tile agg_wide",
     ]
     for e in entries:
         lines.append(" ".join(str(v) for v in e))
@@ -182,7 +199,8 @@ def write_report(report, path: Path, command: str) -> None:
     for (W, H, Dp, D, census, paths, ta, tb, ca, cb, best) in report:
         lines.append(f"| {W}x{H} | {Dp} ({D}) | {'census' if census else 'SAD'} | {paths} | "
                      f"{ta:.3f} | {tb:.3f} | {ca:.3f} | {cb:.3f} | pf {best['agg_prefetch']}, "
-                     f"prio {best['agg_priority']}, tile {best['census_tile']} |")
+                     f"prio {best['agg_priority']}, tile {best['census_tile']}, "
+                     f"wide {best['agg_wide']} |")
     path.write_text("\n".join(lines) + "\n")
 
 

@@ -13,11 +13,14 @@ pytestmark = pytest.mark.gpu
 
 CASES = [
     # (W, H, D, census, paths)
-    (96, 64, 64, False, 4),     # K = 4
-    (130, 70, 128, True, 8),    # K = 8
+    (96, 64, 64, False, 4),     # K = 4 (32-lane scanlines: 2 per lane)
+    (130, 70, 128, True, 8),    # K = 8 (32-lane: 4 per lane)
     (120, 60, 170, True, 4),    # K = 12, padded
     (150, 50, 290, False, 4),   # K = 20, padded
     (100, 40, 384, True, 8),    # K = 24
+    (60, 90, 40, True, 8),      # K = 4 padded (40 -> 64), taller than wide
+    (100, 33, 250, False, 8),   # K = 16 padded (32-lane: 8 per lane)
+    (70, 41, 512, True, 4),     # K = 32 (32-lane: 16 per lane)
 ]
 
 
@@ -33,12 +36,41 @@ def test_every_candidate_schedule_is_bit_identical(gpu_available, case):
     np.testing.assert_array_equal(ref, oracle.sgbm(p, left, right))
     tiles = (128, 256, 512) if census else (256,)
     n = 0
-    for (pf, prio), tile in itertools.product(tune.agg_candidates(K), tiles):
-        m.setSchedule({"agg_prefetch": pf, "agg_priority": prio, "census_tile": tile})
+    for (pf, prio), tile, wide in itertools.product(tune.agg_candidates(K), tiles,
+                                                    tune.wide_settings(K)):
+        m.setSchedule({"agg_prefetch": pf, "agg_priority": prio, "census_tile": tile,
+                       "agg_wide": wide})
         np.testing.assert_array_equal(m.compute(left, right), ref,
-                                      err_msg=f"pf {pf} prio {prio} tile {tile}")
+                                      err_msg=f"pf {pf} prio {prio} tile {tile} wide {wide}")
         n += 1
     assert n >= 6
+
+
+@pytest.mark.parametrize("case", [(96, 64, 0, 64, 0, 4, {}), (130, 77, 0, 40, 1, 8, {}),
+                                  (64, 150, 3, 131, 0, 8, {}), (100, 70, 0, 32, 1, 8,
+                                                                 {"P1": 30000, "P2": 60000}),
+                                  (90, 50, 0, 256, 1, 4, {}), (80, 30, 0, 512, 0, 8, {})],
+                         ids=lambda c: f"{c[0]}x{c[1]}_d{c[2]}-{c[3]}_m{c[4]}_p{c[5]}")
+def test_wide_scanlines_every_path_equals_oracle(gpu_available, case):
+    """Every path on 32-lane scanlines: each path's aggregated costs equal the oracle's (wide
+    scanlines, padded ranges, taller-than-wide frames, minDisparity > 0, forced uint16 wrap)."""
+    W, H, minD, maxD, mode, paths, extra = case
+    p = ls.StereoParams(minDisparity=minD, maxDisparity=maxD, numDirections=paths,
+                        matchingCostMode=ls.MatchingCostMode(mode), uniquenessRatio=0,
+                        speckleWindowSize=0)
+    for k, v in extra.items():
+        setattr(p, k, v)
+    left, right = ls.synth_pair(W, H, max(minD, 2), maxD - 4, seed=W * 31 + H)
+    want, st = oracle.sgbm(p, left, right, dumps=True)
+    m = ls.StereoSGBM(p)
+    for wide in (1, 2, 3):
+        m.setSchedule({"agg_prefetch": 8, "agg_priority": 1, "census_tile": 256,
+                       "agg_wide": wide})
+        got = m.compute(left, right)
+        for r in range(m.numPaths()):
+            np.testing.assert_array_equal(m.dump("path", r), st["paths"][r],
+                                          err_msg=f"wide {wide} path {oracle.PATH_NAMES[r]}")
+        np.testing.assert_array_equal(got, want, err_msg=f"wide {wide} output")
 
 
 def test_unsupported_schedule_is_refused(gpu_available):
@@ -50,6 +82,10 @@ def test_unsupported_schedule_is_refused(gpu_available):
         m.compute(left, right)
     m.setSchedule({"agg_prefetch": 8, "agg_priority": 1, "census_tile": 100})
     with pytest.raises(ls.StereoException, match="census tile"):
+        m.compute(left, right)
+    # 320 padded disparities cannot be split over 32 lanes in whole chunks.
+    m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 1})
+    with pytest.raises(ls.StereoException, match="32-lane"):
         m.compute(left, right)
     m.setSchedule(None)
     np.testing.assert_array_equal(m.compute(left, right), oracle.sgbm(p, left, right))
