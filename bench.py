@@ -5,26 +5,30 @@ Workload: one 1920x1080 uint8 stereo pair, D=128 (minDisparity 0), Census 5x5 co
 paths, P1=200, P2=800 with adaptive P2, WTA + 1/16-px parabola, LR check (1 px), 3x3 median,
 uniqueness 0, speckle off. Synthetic planes-plus-occluders scene (disparities in [4,124]),
 generated deterministically on the host and uploaded once: the timed region starts with the
-inputs resident in HBM. A step is one full compute of the pair. (--workload picks one of two
-other single-pair configurations for profiling: 640x480 D=64 SAD 4-path, 2880x1988 D=290
-Census 4-path.)
+inputs resident in HBM. A step is one full compute of the pair.
 
 value = MDE/s = W*H*D * steps / timed seconds / 1e6 (whole job; one GPU).
 roofline = the dominant kernel (all-paths SGM aggregation): algorithmic bytes per launch =
 W*H*D * 3 * paths (read the uint8 cost once per path, write each uint16 path cost once), divided
 by its mean launch time from hipEvents recorded around it on the stream it runs on.
-cpu_baseline = the oracle's algorithm (oracle/sgbm_oracle.cpp, paths on 8 threads) on the same
-frame; OpenCV is not available on this image, so the OpenCV StereoSGBM baseline cannot run.
+cpu_baseline = the oracle's algorithm (oracle/sgbm_oracle.cpp) threaded over the host cores of
+this process's affinity mask (at most 16), timed as tools/compare_opencv.cpp times OpenCV
+StereoSGBM (one warm-up, mean of 5 runs); OpenCV is not available on this image.
 
-Two more objects in the same line measure the other forms on the uneven 64-pair mix
-(lumenstereo_amd/workloads.py):
-  batched  - pairs/s and MDE/s of the batched executor against the per-pair loop on the same GPU,
-             with host-memory and with device-resident images, and how many of the 64 outputs
-             are bit-identical to the per-pair path's;
-  serving  - an open-loop Poisson sweep (20..400 pairs/s; the mix plus 10% 2880x1988 D=192
-             requests): p50/p99/p99.9 latency and sustained pairs/s of the server against the
-             per-pair path serving the same arrival trace in order, and bit-identity of every
-             served result.
+More objects in the same line:
+  host_images - the same frame through lumen_sgbm_compute on host images, with the reference's
+             getLastComputeTimeMs (upload through post-processing) and the wall time per call;
+  configs  - the other single-pair configurations, 640x480 D=64 SAD 4-path and 2880x1988 D=290
+             Census 4-path: ms per frame, frames/s, stage times, aggregation and frame roofline
+             fractions, and the pixel-wise comparison with the oracle;
+  scene    - the rectification, depth and colour stages either side of the matcher;
+  batched  - pairs/s and MDE/s of the batched executor against the per-pair loop on the same GPU
+             over the uneven 64-pair mix (lumenstereo_amd/workloads.py), with host-memory and with
+             device-resident images, its roofline fraction, and how many of the 64 outputs are
+             bit-identical to the per-pair path's;
+  serving  - an open-loop Poisson sweep (the mix plus 10% 2880x1988 D=192 requests): p50/p99/
+             p99.9 latency and sustained pairs/s of the server against the per-pair path serving
+             the same arrival trace in order, and bit-identity of every served result.
 """
 from __future__ import annotations
 
@@ -65,26 +69,6 @@ def load_traffic(workload):
         return json.loads(f.read_text()).get("hbm_bytes_per_launch")
     except (ValueError, OSError):
         return None
-
-
-def cpu_baseline(ls, params, left, right, gpu_out, threads, desc):
-    import oracle
-
-    t0 = time.perf_counter()
-    ref = oracle.sgbm_mt(params, left, right, threads)
-    dt = time.perf_counter() - t0
-    H, W = left.shape
-    de = W * H * (params.maxDisparity - params.minDisparity)
-    mismatches = int(np.count_nonzero(ref != gpu_out))
-    return {
-        "value": round(de / dt / 1e6, 3),
-        "unit": "MDE/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"one full {desc} frame, oracle algorithm with the paths on {threads} threads "
-                  f"({dt:.2f} s); OpenCV StereoSGBM unavailable",
-        "seconds": round(dt, 3),
-    }, mismatches
 
 
 def _pct(v, q):
@@ -146,9 +130,15 @@ def bench_batched(ls, torch, specs, imgs, plain_outs, reps):
     gc.collect()
     torch.cuda.empty_cache()
 
+    # Algorithmic HBM bytes of the mix: W*H*D*(1 + 5*paths) per pair (the uint8 cost volume
+    # written once, read once per path; each uint16 path cost written once and read once).
+    alg_bytes = sum(s.de * (1 + 5 * (8 if s.params.numDirections >= 8 else 4)) for s in specs)
+    t_roof = alg_bytes / (HBM_PEAK_GBS * 1e9)
+
     def rate(t):
         return {"pairs_per_s": round(n / t, 1), "MDE_per_s": round(total_de / t / 1e6, 1),
-                "ms_per_batch": round(t * 1e3, 3)}
+                "ms_per_batch": round(t * 1e3, 3),
+                "roofline_frac": round(t_roof / t, 4)}
 
     return {
         "workload": f"{n} uneven pairs (W in {{320,640,1280,1920}}, H 9/16 or 3/4 W, D in "
@@ -160,6 +150,12 @@ def bench_batched(ls, torch, specs, imgs, plain_outs, reps):
         "batched_device_io": rate(t_batch_dev),
         "speedup_host_io": round(t_plain_host / t_batch_host, 3),
         "speedup_device_io": round(t_plain_dev / t_batch_dev, 3),
+        "algorithmic_GB": round(alg_bytes / 1e9, 2),
+        "roofline_ms_per_batch": round(t_roof * 1e3, 3),
+        "speedup_ceiling_device_io": round(t_plain_dev / t_roof, 3),
+        "speedup_ceiling_note": "the per-pair loop's time over the mix's HBM roofline time: no "
+                                "schedule of this dataflow (1 + 5*paths bytes per disparity "
+                                "evaluation, all algorithmic) can beat the per-pair loop by more",
         "batched_host_split_ms": {"total": host_split[0], "upload": host_split[1],
                                   "compute": host_split[2], "download": host_split[3]},
         "bit_identical_pairs": f"{identical}/{n} host, {dev_identical}/{n} device",
@@ -266,6 +262,12 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
         print(f"# rate {rate:g}/s: {N} requests, served p99 {_pct(lats, 99)} ms, plain p99 "
               f"{_pct(plat, 99)} ms", file=sys.stderr, flush=True)
         alone_tr = [alone[e] for _, e in trace]
+        slow = [lat / alone[e] for lat, (_, e) in zip(lats, trace)]
+
+        def phase(a, b):
+            v = [(i[b] - i[a]) / 1e6 for i in infos]
+            return {"p50_ms": _pct(v, 50), "p99_ms": _pct(v, 99)}
+
         sweep.append({
             "rate": rate, "requests": N,
             "served": {"p50_ms": _pct(lats, 50), "p99_ms": _pct(lats, 99),
@@ -275,7 +277,35 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
                       "p999_ms": _pct(plat, 99.9), "pairs_per_s": round(N / pspan, 1)},
             "p99_served_over_plain": round(_pct(lats, 99) / _pct(plat, 99), 3),
             "p99_served_over_alone_p99": round(_pct(lats, 99) / _pct(alone_tr, 99), 3),
+            "slowdown_vs_alone": {"p50": _pct(slow, 50), "p99": _pct(slow, 99)},
+            "served_phases": {"submit_to_admit": phase("t_submit_ns", "t_admit_ns"),
+                              "admit_to_start": phase("t_admit_ns", "t_start_ns"),
+                              "start_to_done": phase("t_start_ns", "t_done_ns")},
         })
+    # Saturation: a closed loop that keeps `depth` requests in flight, against the per-pair
+    # path computing the same requests back to back.
+    depth, n_sat = 24, max(400, int(rates[-1] * seconds)) if rates else 400
+    order = [int(rng.choice(big)) if rng.random() < 0.1 else int(rng.integers(0, len(specs)))
+             for _ in range(n_sat)]
+    t0 = time.perf_counter()
+    live = []
+    for e in order:
+        live.append(srv.submit(*entries[e]))
+        if len(live) >= depth:
+            srv.wait(live.pop(0))
+    for t in live:
+        srv.wait(t)
+    t_srv = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for e in order:
+        p, l, r = entries[e]
+        m.setParams(p)
+        m.compute(l, r)
+    t_plain = time.perf_counter() - t0
+    saturation = {"requests": n_sat, "in_flight": depth,
+                  "served_pairs_per_s": round(n_sat / t_srv, 1),
+                  "plain_pairs_per_s": round(n_sat / t_plain, 1),
+                  "served_over_plain": round(t_plain / t_srv, 3)}
     st = srv.stats()
     srv.close()
     del srv, m
@@ -289,9 +319,11 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
         "plain_alone_ms": {"p50": _pct(alone, 50), "p99": _pct(alone, 99),
                            "mix_mean": round(float(np.mean(alone[:len(specs)])), 3),
                            "big_mean": round(float(np.mean([alone[e] for e in big])), 3)},
+        "saturation": saturation,
         "bit_identical": f"{total - mismatched}/{total}",
         "server_stats": {k: st[k] for k in ("batches", "max_batch_seen", "max_lanes_busy",
-                                            "pool_high_water", "max_waiting")},
+                                            "pool_high_water", "max_waiting", "direct_starts",
+                                            "unstaged")},
         "seconds_per_rate": seconds,
     }
 
@@ -364,19 +396,149 @@ def bench_scene(ls, torch, W, H, reps):
     return out
 
 
+def cpu_threads_available():
+    """Host cores this process may run on (its affinity mask)."""
+    try:
+        import os
+
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        import os
+
+        return os.cpu_count() or 1
+
+
+def bench_single(ls, torch, name, steps, warmup, cpu_threads, cpu_runs, parity=True):
+    """One single-pair configuration of workloads.SINGLE: `steps` computes from device images
+    (inputs resident in HBM) timed by the host clock between two device synchronisations, the
+    stage times from hipEvents on the handle's stream (mean over the timed computes), the
+    aggregation's roofline fraction, and optionally the CPU baseline and the pixel-wise
+    comparison of the GPU output with the oracle's on the same frame."""
+    from lumenstereo_amd import workloads
+
+    W, H, D, census, paths, _, _ = workloads.SINGLE[name]
+    params, left, right = workloads.single_pair(name)
+    dev = torch.device("cuda:0")
+    d_left = torch.from_numpy(left).to(dev)
+    d_right = torch.from_numpy(right).to(dev)
+    d_out = torch.empty((H, W), dtype=torch.int16, device=dev)
+    m = ls.StereoSGBM(params)
+    npaths = m.numPaths()
+    for _ in range(warmup):
+        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stage = m.timings(ncalls=min(steps, 64))  # total, cost, agg, post, wta, upload
+    de = W * H * D
+    ms = elapsed * 1e3 / steps
+    agg_ms = float(stage[2])
+    agg_bytes = de * 3 * npaths
+    frame_bytes = de * (1 + 5 * npaths)
+    out = {
+        "workload": f"plain single pair {W}x{H} uint8, D={D}, "
+                    f"{'Census 5x5' if census else 'Sobel-x + SAD 5x5'}, {npaths} paths, "
+                    "P1=200 P2=800 adaptive, WTA+subpixel, LR check, 3x3 median",
+        "width": W, "height": H, "disparities": D, "cost": "census" if census else "sad",
+        "paths": npaths, "seed": SEED,
+        "ms_per_frame": round(ms, 4),
+        "frames_per_s": round(1e3 / ms, 2),
+        "MDE_per_s": round(de / (ms * 1e-3) / 1e6, 1),
+        "stage_ms": {"cost": round(float(stage[1]), 4), "aggregation": round(agg_ms, 4),
+                     "sum_wta_lr": round(float(stage[4]), 4), "post": round(float(stage[3]), 4),
+                     "pipeline": round(float(stage[0]), 4)},
+        "aggregation_GB_per_s": round(agg_bytes / (agg_ms * 1e-3) / 1e9, 1),
+        "aggregation_roofline_frac": round(agg_bytes / (agg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "frame_roofline_frac": round(frame_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "schedule": m.getSchedule(W, H),
+        "_agg_bytes": agg_bytes,
+    }
+    gpu_out = d_out.cpu().numpy()
+    if parity or cpu_threads:
+        import oracle
+
+        desc = f"{W}x{H} D={D} {'Census' if census else 'SAD'} {npaths}-path"
+        threads = max(1, cpu_threads or 16)
+        # The timing of tools/compare_opencv.cpp:279-287 for the CPU matcher: one warm-up run,
+        # then the mean of `cpu_runs` runs by the host clock.
+        ref = oracle.sgbm(params, left, right, threads=threads)
+        out["parity_vs_oracle_mismatched_pixels"] = int(np.count_nonzero(ref != gpu_out))
+        if cpu_threads:
+            t0 = time.perf_counter()
+            for _ in range(cpu_runs):
+                oracle.sgbm(params, left, right, threads=threads)
+            dt = (time.perf_counter() - t0) / cpu_runs
+            out["cpu_baseline"] = {
+                "value": round(de / dt / 1e6, 3),
+                "unit": "MDE/s",
+                "cores": threads,
+                "kind": "port",
+                "sample": f"one full {desc} frame: the oracle's algorithm threaded over "
+                          f"{threads} host threads (cost, path sum and WTA in bands of rows, "
+                          f"one path per thread), one warm-up run then the mean of {cpu_runs} "
+                          f"runs ({dt:.2f} s per frame), as tools/compare_opencv.cpp times "
+                          "OpenCV StereoSGBM, which is not installed here",
+                "seconds_per_frame": round(dt, 3),
+                "cores_available": cpu_threads_available(),
+            }
+    del m, d_left, d_right, d_out
+    gc.collect()
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_host_frame(ls, name, runs):
+    """The reference's own timing of a frame with host images (StereoSGBM::compute on cv::Mat,
+    src/sgbm_gpu.cu:177-191): getLastComputeTimeMs spans the upload through post-processing on
+    the handle's stream; the wall time adds the download and the copies between the caller's
+    pageable buffers and the pinned staging. One warm-up call, then the mean of `runs` calls,
+    as tools/compare_opencv.cpp:292-300 does."""
+    from lumenstereo_amd import workloads
+
+    params, left, right = workloads.single_pair(name)
+    H, W = left.shape
+    D = params.maxDisparity - params.minDisparity
+    m = ls.StereoSGBM(params)
+    m.compute(left, right)
+    last, wall = [], []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        m.compute(left, right)
+        wall.append((time.perf_counter() - t0) * 1e3)
+        last.append(m.getLastComputeTimeMs())
+    ms = float(np.mean(last))
+    return {
+        "workload": f"{W}x{H} D={D}: lumen_sgbm_compute on host images (the reference's "
+                    "compute(cv::Mat...) form)",
+        "getLastComputeTimeMs": round(ms, 4),
+        "MDE_per_s_by_getLastComputeTimeMs": round(W * H * D / (ms * 1e-3) / 1e6, 1),
+        "wall_ms_per_frame": round(float(np.mean(wall)), 4),
+        "frames_per_s_wall": round(1e3 / float(np.mean(wall)), 2),
+        "timing": f"one warm-up, mean of {runs} calls (tools/compare_opencv.cpp:292-300)",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="host threads of the CPU baseline (0: the affinity mask's cores, at "
+                         "most 16)")
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--skip-forms", action="store_true",
                     help="only the single-pair workload (no batched / serving measurement)")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--skip-scene", action="store_true",
                     help="no measurement of the rectification / depth / colour stages")
+    ap.add_argument("--skip-configs", action="store_true",
+                    help="no measurement of the other single-pair configurations")
     ap.add_argument("--scene-size", default="2880x1988",
                     help="image size of the rectification / depth / colour measurement")
     ap.add_argument("--mix-pairs", type=int, default=64)
@@ -397,68 +559,43 @@ def main() -> int:
 
     import lumenstereo_amd as ls
 
-    W, H, D, census, paths, (dmin, dmax), lowtex = WORKLOADS[args.workload]
-    params = workload_params(ls, D, census, paths)
-    left, right = ls.synth_pair(W, H, dmin, dmax, SEED, lowtex)
-    dev = torch.device("cuda:0")
-    d_left = torch.from_numpy(left).to(dev)
-    d_right = torch.from_numpy(right).to(dev)
-    d_out = torch.empty((H, W), dtype=torch.int16, device=dev)
-    m = ls.StereoSGBM(params)
-    npaths = m.numPaths()
-
-    for _ in range(args.warmup):
-        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    stage = m.timings(ncalls=min(args.steps, 64))  # total, cost, agg, post, wta, upload
-
+    cpu_threads = 0 if args.no_cpu_baseline else (args.cpu_threads or
+                                                   min(cpu_threads_available(), 16))
+    W, H, D, census, paths, _, _ = WORKLOADS[args.workload]
+    single = bench_single(ls, torch, args.workload, args.steps, args.warmup, cpu_threads,
+                          args.cpu_runs, parity=not args.no_cpu_baseline)
     de = W * H * D
-    ms_per_step = elapsed * 1e3 / args.steps
-    value = de * args.steps / elapsed / 1e6
-    agg_ms = float(stage[2])
-    agg_bytes = de * 3 * npaths
+    ms_per_step = single["ms_per_frame"]
+    agg_bytes = single.pop("_agg_bytes")
+    agg_ms = single["stage_ms"]["aggregation"]
     achieved = agg_bytes / (agg_ms * 1e-3) / 1e9
-    frame_bytes = de * (1 + 5 * npaths)
 
     if args.copy_calibration_mb > 0:
         n = args.copy_calibration_mb << 20
-        src = torch.empty(n, dtype=torch.uint8, device=dev).fill_(1)
+        src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
 
-    gpu_out = d_out.cpu().numpy()
     line = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": single["MDE_per_s"],
         "unit": "MDE/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "uint16",
         "data": "synthetic",
-        "config": {
-            "workload": f"plain single pair {W}x{H} uint8, D={D}, "
-                        f"{'Census 5x5' if census else 'Sobel-x + SAD 5x5'}, {npaths} paths, "
-                        "P1=200 P2=800 adaptive, WTA+subpixel, LR check, 3x3 median",
-            "width": W, "height": H, "disparities": D, "cost": "census" if census else "sad",
-            "paths": npaths, "seed": SEED,
-        },
-        "frames_per_s": round(1e3 / ms_per_step, 2),
-        "stage_ms": {"cost": round(float(stage[1]), 4), "aggregation": round(agg_ms, 4),
-                     "sum_wta_lr": round(float(stage[4]), 4), "post": round(float(stage[3]), 4),
-                     "pipeline": round(float(stage[0]), 4)},
+        "config": {k: single[k] for k in ("workload", "width", "height", "disparities", "cost",
+                                          "paths", "seed")},
+        "frames_per_s": single["frames_per_s"],
+        "stage_ms": single["stage_ms"],
         "roofline": {
             "bound": "hbm",
             "kernel": "sgm_paths_kernel (all paths, one launch)",
@@ -469,22 +606,32 @@ def main() -> int:
             "traffic": load_traffic(args.workload),
             "algorithmic_bytes_per_launch": agg_bytes,
         },
-        "frame_roofline_frac": round(frame_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "frame_roofline_frac": single["frame_roofline_frac"],
+        "schedule": single["schedule"],
         "reference_published": {"value": 664, "unit": "MDE/s",
                                 "note": "README: 1920x1080 D=128 in 400 ms on an RTX 4050 Laptop "
                                         "GPU; cost mode and path count not stated"},
     }
-    if not args.no_cpu_baseline:
-        desc = f"{W}x{H} D={D} {'Census' if census else 'SAD'} {npaths}-path"
-        cb, mism = cpu_baseline(ls, params, left, right, gpu_out, args.cpu_threads, desc)
-        line["cpu_baseline"] = cb
-        line["parity_vs_oracle_mismatched_pixels"] = mism
+    if "cpu_baseline" in single:
+        line["cpu_baseline"] = single["cpu_baseline"]
+    if "parity_vs_oracle_mismatched_pixels" in single:
+        line["parity_vs_oracle_mismatched_pixels"] = single["parity_vs_oracle_mismatched_pixels"]
+    if not args.skip_configs and args.workload == DEFAULT_WORKLOAD:
+        print("# host-image frame", file=sys.stderr, flush=True)
+        line["host_images"] = bench_host_frame(ls, args.workload, 5)
+        line["configs"] = {}
+        for name in WORKLOADS:
+            if name == args.workload:
+                continue
+            print(f"# config {name}", file=sys.stderr, flush=True)
+            c = bench_single(ls, torch, name, 20, 3, 0, 0, parity=not args.no_cpu_baseline)
+            c.pop("_agg_bytes")
+            line["configs"][name] = c
     if not args.skip_scene:
         sw, sh = (int(v) for v in args.scene_size.lower().split("x"))
         print(f"# scene stages {sw}x{sh}", file=sys.stderr, flush=True)
         line["scene"] = bench_scene(ls, torch, sw, sh, 20)
     if not args.skip_forms:
-        del m, d_left, d_right, d_out
         gc.collect()
         torch.cuda.empty_cache()
         from lumenstereo_amd import workloads

@@ -159,8 +159,7 @@ typedef struct lumen_schedule {
                               half the disparities per lane: a shorter dependency chain per
                               step for twice the waves) instead of 16 (four per wave): 0 none,
                               1 the straight pair along the longer image dimension, 2 all four
-                              straight paths, 3 every path. Non-zero only for padded ranges 64,
-                              128, 256 and 512. */
+                              straight paths, 3 every path. */
 } lumen_schedule;
 
 /* Overrides the tuned schedule for this handle (NULL restores it). Checked at compute time:
@@ -229,11 +228,14 @@ const char* lumen_batch_last_error(const lumen_batch* b);
 
 /* ------------------------------------------------------------------------------------------
  * Serving form (no reference counterpart). Requests (stereo pairs of any size, range, cost mode
- * and path count) are submitted as they arrive and return a ticket. A dispatcher thread admits
- * them into a device pool by their working-set size (cost volume, path buffers, maps), and
- * whenever one of the lanes (a stream with its own persistent-kernel pipeline) is free it starts
- * admitted requests of similar size as one batch on it. So requests that arrive while others
- * are on the GPU run alongside them. Every result is bit-identical to
+ * and path count) are submitted as they arrive and return a ticket. Requests are admitted into
+ * a device pool by their working-set size (cost volume, path buffers, maps), and whenever one
+ * of the lanes (a stream with its own persistent-kernel pipeline) is free, admitted requests of
+ * similar size start on it as one batch: from the submitting thread when a lane is idle and
+ * nothing is waiting, otherwise from a dispatcher thread. So requests that arrive while others
+ * are on the GPU run alongside them. Host images are staged into a pinned block of the server
+ * by the submitting thread, and a waiter copies its output from the pinned block itself, so
+ * the dispatcher never copies host memory. Every result is bit-identical to
  * lumen_sgbm_compute of the same pair with the same parameters.
  * ---------------------------------------------------------------------------------------- */
 typedef struct lumen_server_config {
@@ -246,6 +248,13 @@ typedef struct lumen_server_config {
                                 algorithmic HBM traffic (W*H*Dp*(1+5*paths) bytes); 0 = 2000.
                                 A batch only groups requests within 4x of each other's work, so
                                 small requests do not wait for a large one. */
+    int32_t pinned_mb;   /* pinned host staging pool in MiB (host-memory requests); 0 = 1024.
+                            A request that finds no room there is staged by the dispatcher
+                            through its lane's own buffers instead. */
+    int32_t large_us;    /* requests whose work (as for batch_budget_us) is above this many
+                            microseconds are large: they run one at a time, in arrival order, on
+                            lane 0, which takes other requests only while no large one is ready;
+                            the other lanes never take a large one. 0 = no large class. */
     int32_t reserved;
 } lumen_server_config;
 
@@ -257,7 +266,7 @@ typedef struct lumen_request_info {
     int64_t t_submit_ns; /* CLOCK_MONOTONIC times (lumen_now_ns) */
     int64_t t_admit_ns;  /* pool range granted */
     int64_t t_start_ns;  /* its batch enqueued */
-    int64_t t_done_ns;   /* output written */
+    int64_t t_done_ns;   /* output in the caller's buffer */
 } lumen_request_info;
 
 typedef struct lumen_server_stats {
@@ -266,6 +275,8 @@ typedef struct lumen_server_stats {
     uint64_t max_lanes_busy;   /* most lanes that had a batch in flight at the same time */
     uint64_t pool_bytes, pool_high_water;
     uint64_t max_waiting;      /* most requests waiting for admission at once */
+    uint64_t direct_starts;    /* batches started by a submitting thread on an idle lane */
+    uint64_t unstaged;         /* host requests that found no room in the pinned pool */
 } lumen_server_stats;
 
 typedef struct lumen_server lumen_server;

@@ -175,6 +175,7 @@ class CServerConfig(ctypes.Structure):
     _fields_ = [("pool_bytes", ctypes.c_size_t), ("lanes", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("device_io", ctypes.c_int32),
                 ("max_bypass", ctypes.c_int32), ("batch_budget_us", ctypes.c_int32),
+                ("pinned_mb", ctypes.c_int32), ("large_us", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
 
 
@@ -192,7 +193,7 @@ class CServerStats(ctypes.Structure):
 
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "submitted", "completed", "failed", "batches", "max_batch_seen", "max_lanes_busy",
-        "pool_bytes", "pool_high_water", "max_waiting")]
+        "pool_bytes", "pool_high_water", "max_waiting", "direct_starts", "unstaged")]
 
 
 class CCameraParams(ctypes.Structure):
@@ -558,12 +559,14 @@ class ServeSGBM:
     Results are bit-identical to StereoSGBM."""
 
     def __init__(self, pool_bytes: int = 0, lanes: int = 2, max_batch: int = 64,
-                 device_io: bool = False, max_bypass: int = 8, batch_budget_us: int = 2000):
+                 device_io: bool = False, max_bypass: int = 8, batch_budget_us: int = 2000,
+                 pinned_mb: int = 0, large_us: int = 0):
         self._lib = library()
         self._h = ctypes.c_void_p()
         self._device_io = bool(device_io)
         cfg = CServerConfig(int(pool_bytes), int(lanes), int(max_batch), int(device_io),
-                            int(max_bypass), int(batch_budget_us), 0)
+                            int(max_bypass), int(batch_budget_us), int(pinned_mb), int(large_us),
+                            0)
         rc = self._lib.lumen_server_create(ctypes.byref(cfg), ctypes.byref(self._h))
         if rc != LUMEN_OK:
             raise GpuException(self._lib.lumen_server_last_error(None).decode(), rc)

@@ -53,15 +53,13 @@ constexpr int kRowRor1 = 0x121;        // lane i reads lane (i-1) mod 16 of its 
 constexpr int kRowRor15 = 0x12F;       // lane i reads lane (i+1) mod 16 of its row
 constexpr int kRowMirror = 0x140;      // lane i reads lane 15-i of its row
 constexpr int kRowHalfMirror = 0x141;  // lane i reads lane 7-i of its half row
-constexpr int kWaveShr1 = 0x138;       // lane i reads lane i-1 of the wave (lane 0: no source)
-constexpr int kWaveShl1 = 0x130;       // lane i reads lane i+1 of the wave (lane 63: no source)
+constexpr int kWaveRor1 = 0x13C;       // lane i reads lane (i-1) mod 64 of the wave
+constexpr int kWaveRol1 = 0x134;       // lane i reads lane (i+1) mod 64 of the wave
 
-// v of the lane CTRL names. The row controls give every lane a source (the ones above and
-// row_newbcast), so no lane keeps an old value and the bound_ctrl form is exact; it also lets
-// the compiler fold the move into the VALU instruction that consumes it (v_min_u32 with a DPP
-// source), one instruction per butterfly stage instead of three. The wave shifts leave lane 0
-// (shr) or lane 63 (shl) without a source: those lanes read 0, and their users mask them (they
-// stand for disparity -1 or Dp).
+// v of the lane CTRL names. Only controls that give every lane a source are used (the ones
+// above and row_newbcast), so no lane keeps an old value and the bound_ctrl form is exact; it
+// also lets the compiler fold the move into the VALU instruction that consumes it (v_min_u32
+// with a DPP source), one instruction per butterfly stage instead of three.
 //
 // Contract: every caller keeps whole 16-lane rows active (no divergent branch around a DPP
 // use that disables part of a row). A DPP read of a disabled source lane returns 0 in the
@@ -609,11 +607,14 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 // and one 8- or 16-byte path-cost access per lane, and the 16 lanes' chunks are contiguous, so
 // every access instruction of a row covers one contiguous 64 to 256 bytes whatever K is. For
 // K <= 8 there is one chunk and lane l simply holds d = lK .. lK+K-1.
-// LPS = 32 (K = 2, 4, 8 or 16): one chunk, lane l holds d = lK .. lK+K-1.
+// LPS = 32: for K = 2, 4, 8 or 16 one chunk, lane l holding d = lK .. lK+K-1; otherwise
+// chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities laid out as above with 32
+// lanes, so each access of a scanline covers 64 to 512 contiguous bytes.
 // Registers hold the lane's elements chunk by chunk, element e being slice_d<K, LPS>(l, e).
 template <int K, int LPS = 16>
 __host__ __device__ constexpr int slice_q() {
-    return LPS == 32 ? K : (K % 8 == 0 ? 8 : 4);
+    return LPS == 32 ? ((K == 2 || K == 4 || K == 8 || K == 16) ? K : (K % 4 == 0 ? 4 : 2))
+                     : (K % 8 == 0 ? 8 : 4);
 }
 
 template <int K, int LPS = 16>
@@ -634,22 +635,22 @@ __device__ __forceinline__ int slice_elem(int d) {
     return d / (16 * Q) * Q + d % Q;
 }
 
-// A lane's K cost bytes, four to a word (for K = 2 the low half of one word).
-template <int K>
+// A lane's K cost bytes: four to a word, or for chunks of two one chunk per word (low half).
+template <int K, int LPS = 16>
 struct CostBytes {
-    uint32_t w[(K + 3) / 4];
+    uint32_t w[slice_q<K, LPS>() == 2 ? K / 2 : K / 4];
 };
 
 // p: the pixel's cost vector + Q*l (the lane's first chunk).
 template <int K, int LPS = 16>
-__device__ __forceinline__ CostBytes<K> load_cost(const LUMEN_G uint8_t* p) {
+__device__ __forceinline__ CostBytes<K, LPS> load_cost(const LUMEN_G uint8_t* p) {
     constexpr int Q = slice_q<K, LPS>();
-    CostBytes<K> c;
+    CostBytes<K, LPS> c;
 #pragma unroll
     for (int ch = 0; ch < K / Q; ++ch) {
         const LUMEN_G uint8_t* q = p + LPS * Q * ch;
         if constexpr (Q == 2) {
-            c.w[0] = *reinterpret_cast<const LUMEN_G uint16_t*>(q);
+            c.w[ch] = *reinterpret_cast<const LUMEN_G uint16_t*>(q);
         } else if constexpr (Q == 4) {
             c.w[ch] = *reinterpret_cast<const LUMEN_G uint32_t*>(q);
         } else if constexpr (Q == 8) {
@@ -693,10 +694,13 @@ __device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)
 }
 
 // uint8 costs -> packed t16(C * costScale).
-template <int K>
-__device__ __forceinline__ void expand_cost(const CostBytes<K>& c, u16x2 cs, u16x2 (&out)[K / 2]) {
-    if constexpr (K == 2) {
-        out[0] = as16(__builtin_amdgcn_perm(0u, c.w[0], 0x0c010c00u)) * cs;
+template <int K, int LPS = 16>
+__device__ __forceinline__ void expand_cost(const CostBytes<K, LPS>& c, u16x2 cs,
+                                            u16x2 (&out)[K / 2]) {
+    if constexpr (slice_q<K, LPS>() == 2) {
+#pragma unroll
+        for (int i = 0; i < K / 2; ++i)
+            out[i] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c010c00u)) * cs;
     } else {
 #pragma unroll
         for (int i = 0; i < K / 4; ++i) {
@@ -706,33 +710,46 @@ __device__ __forceinline__ void expand_cost(const CostBytes<K>& c, u16x2 cs, u16
     }
 }
 
+// The value lane i + 32 (for i < 32) or i - 32 holds: the other half of the wave.
+__device__ __forceinline__ uint32_t other_half(uint32_t x, bool upper) {
+    const auto sw = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return upper ? static_cast<uint32_t>(sw[0]) : static_cast<uint32_t>(sw[1]);
+}
+
 // d-1 and d+1 neighbour vectors of a packed slice (lane l of its scanline's lanes). Inside a
-// chunk they are the lane's own neighbouring elements. 16-lane rows: at a chunk's start, d-1 is
-// the last element of the same chunk of lane l-1, or for lane 0 the last element of the
-// previous chunk of lane 15; at a chunk's end, d+1 is the first element of the same chunk of
-// lane l+1, or for lane 15 of the next chunk of lane 0: one row rotation per chunk each way,
-// then a lane select. 32-lane scanlines (one chunk): the neighbours come from lanes l-1 and l+1
-// through whole-wave shifts, which cross the boundary between the two DPP rows. The values lane
+// chunk they are the lane's own neighbouring elements. At a chunk's start, d-1 is the last
+// element of the same chunk of lane l-1, or for lane 0 the last element of the previous chunk
+// of the scanline's last lane; at a chunk's end, d+1 is the first element of the same chunk of
+// lane l+1, or for the last lane of the next chunk of lane 0. 16-lane rows: one row rotation
+// per chunk each way, then a lane select. 32-lane scanlines: whole-wave rotations by one lane
+// (which cross the boundary between the two DPP rows of a scanline); the wrap from a
+// scanline's last lane to its lane 0 is the rotation's value in the other half of the wave
+// (lane 32 holds lane 31's, lane 0 lane 63's), fetched with v_permlane32_swap. The values lane
 // 0 gets for d = -1 and the last lane for d = Dp are masked by the caller.
 template <int K, int LPS = 16>
 __device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&dm1)[K / 2],
                                            uint32_t (&dp1)[K / 2], int l) {
     constexpr int Q = slice_q<K, LPS>(), NC = K / Q, PQ = Q / 2;
-    uint32_t prevLast[NC], nextFirst[NC];
+    uint32_t prevLast[NC], nextFirst[NC], prevWrap[NC], nextWrap[NC];
+    const bool upper = (threadIdx.x & 63) >= 32;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         if constexpr (LPS == 32) {
-            prevLast[c] = dpp<kWaveShr1>(bits(v[c * PQ + PQ - 1]));
-            nextFirst[c] = dpp<kWaveShl1>(bits(v[c * PQ]));
+            prevLast[c] = dpp<kWaveRor1>(bits(v[c * PQ + PQ - 1]));
+            nextFirst[c] = dpp<kWaveRol1>(bits(v[c * PQ]));
+            if (c < NC - 1) prevWrap[c] = other_half(prevLast[c], upper);
+            if (c > 0) nextWrap[c] = other_half(nextFirst[c], upper);
         } else {
             prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]));
             nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]));
+            prevWrap[c] = prevLast[c];
+            nextWrap[c] = nextFirst[c];
         }
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const uint32_t fromPrev = (c > 0 && l == 0) ? prevLast[c - 1] : prevLast[c];
-        const uint32_t fromNext = (c < NC - 1 && l == LPS - 1) ? nextFirst[c + 1] : nextFirst[c];
+        const uint32_t fromPrev = (c > 0 && l == 0) ? prevWrap[c - 1] : prevLast[c];
+        const uint32_t fromNext = (c < NC - 1 && l == LPS - 1) ? nextWrap[c + 1] : nextFirst[c];
 #pragma unroll
         for (int jj = 0; jj < PQ; ++jj) {
             const int q = c * PQ + jj;
@@ -800,8 +817,7 @@ __device__ __forceinline__ uint32_t p2_lookup(const P2Table& t, uint32_t g) {
 template <int K, bool PAD, int PF = default_prefetch(K), int LPS = 16>
 __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_base) {
     static_assert(LPS == 16 || LPS == 32, "a scanline spans one or two DPP rows");
-    static_assert(LPS == 16 || K == 2 || K == 4 || K == 8 || K == 16,
-                  "32-lane scanlines hold 2, 4, 8 or 16 disparities per lane");
+    static_assert(K % 2 == 0, "a lane holds whole packed pairs of disparities");
     static_assert(valid_prefetch(K, PF, LPS), "unsupported prefetch depth for this slice");
     const int lane = threadIdx.x & 63;
     const int g = lane / LPS, l = lane & (LPS - 1);
@@ -840,8 +856,8 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     LUMEN_G uint16_t* const trash = gmem(trash_base) + g * LPS * K + l * Q;
     u16x2 Lv[K / 2];
     {
-        const CostBytes<K> c0 = load_cost<K, LPS>(lp);
-        expand_cost<K>(c0, cs, Lv);
+        const CostBytes<K, LPS> c0 = load_cost<K, LPS>(lp);
+        expand_cost<K, LPS>(c0, cs, Lv);
         if constexpr (PAD) {
 #pragma unroll
             for (int q = 0; q < K / 2; ++q) Lv[q] = as16(bits(Lv[q]) | padm[q]);
@@ -861,7 +877,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     // or 64-bit selects).
     const int last = len > 0 ? len - 1 : 0;
     const int lu = l & (PF - 1);
-    auto fetch = [&](auto checked, int tb, CostBytes<K>(&dst)[PF], int (&iv)[2]) {
+    auto fetch = [&](auto checked, int tb, CostBytes<K, LPS>(&dst)[PF], int (&iv)[2]) {
         if constexpr (decltype(checked)::value) {
             const int ti = min(tb + lu, last);
             iv[0] = gmem(j.left)[p0 + ti * step];
@@ -885,7 +901,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
             lt += PF;
         }
     };
-    auto run = [&](auto checked, int tb, const CostBytes<K>(&blk)[PF], const int (&iv)[2]) {
+    auto run = [&](auto checked, int tb, const CostBytes<K, LPS>(&blk)[PF], const int (&iv)[2]) {
         const uint32_t p2v = p2_lookup(p2t, static_cast<uint32_t>(abs(iv[0] - iv[1])));
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -893,7 +909,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
             const u16x2 c3 = splat(M + row_bcast(p2v, u));
             const u16x2 Mp = splat(M);
             u16x2 Ch[K / 2];
-            expand_cost<K>(blk[u], cs, Ch);
+            expand_cost<K, LPS>(blk[u], cs, Ch);
             uint32_t dm1[K / 2], dp1[K / 2];
             neighbours<K, LPS>(Lv, dm1, dp1, l);
 #pragma unroll
@@ -920,7 +936,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     };
     using Checked = std::integral_constant<bool, true>;
     using Unchecked = std::integral_constant<bool, false>;
-    CostBytes<K> bufA[PF], bufB[PF];
+    CostBytes<K, LPS> bufA[PF], bufB[PF];
     int gA[2], gB[2];
     fetch(Checked{}, 1, bufA, gA);
     // Entering the loop with block A complete keeps the loop-header state identical on both

@@ -20,10 +20,11 @@ constexpr int kTrashElems = 64 * 32;
 inline int padded_range(int D) { return (D + 63) / 64 * 64; }
 
 // Vector stores per aggregation step and lane. 16-lane scanlines: K/8 16-byte stores, or K/4
-// 8-byte ones. 32-lane scanlines (K = 2, 4, 8, 16 consecutive disparities per lane): one 4-,
-// 8- or 16-byte store, or two 16-byte ones for K = 16.
+// 8-byte ones. 32-lane scanlines: one 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones
+// (K = 16), K/4 8-byte ones (K = 12) or K/2 4-byte ones (K = 6, 10, 14).
 __host__ __device__ constexpr int path_stores(int K, int LPS = 16) {
-    return LPS == 32 ? (K <= 8 ? 1 : K / 8) : (K % 8 == 0 ? K / 8 : K / 4);
+    return LPS == 32 ? (K == 2 || K == 4 || K == 8 ? 1 : K == 16 ? 2 : K % 4 == 0 ? K / 4 : K / 2)
+                     : (K % 8 == 0 ? K / 8 : K / 4);
 }
 
 // Prefetch depths the aggregation supports for slice K: each lane of a DPP row fetches the
@@ -33,10 +34,10 @@ __host__ __device__ constexpr bool valid_prefetch(int K, int PF, int LPS = 16) {
     return (PF == 4 || PF == 8 || PF == 16) && PF * path_stores(K, LPS) < 64;
 }
 
-// Lanes per scanline the aggregation supports for a padded range Dp: 16 always; 32 when each
-// lane then holds 2, 4, 8 or 16 disparities (Dp = 64, 128, 256, 512).
-__host__ __device__ constexpr bool wide_scanlines_ok(int Dp) {
-    return Dp == 64 || Dp == 128 || Dp == 256 || Dp == 512;
+// Whether 32-lane scanlines can run slice K (disparities per 16 lanes) with prefetch depth PF:
+// each lane then holds K/2 disparities, and a block's stores must fit the wait count.
+__host__ __device__ constexpr bool wide_scanlines_ok(int K, int PF) {
+    return K % 2 == 0 && valid_prefetch(K / 2, PF, 32);
 }
 
 __host__ __device__ constexpr int default_prefetch(int K) { return K <= 8 ? 8 : 4; }
@@ -47,9 +48,9 @@ struct Schedule {
     int aggPrefetch;  // aggregation cost prefetch depth: 4, 8 or 16 (valid_prefetch)
     int aggPriority;  // aggregation wave priority: 0 none, 1 by path kind, 2 by scanline length
     int censusTile;   // census cost segment width: 128, 256 or 512 pixels
-    int aggWide;      // paths on 32-lane scanlines (two per wave, wide_scanlines_ok ranges):
-                      // 0 none, 1 the straight pair along the longer image dimension,
-                      // 2 all four straight paths, 3 every path; the others run 16-lane ones
+    int aggWide;      // paths on 32-lane scanlines (two per wave): 0 none, 1 the straight
+                      // pair along the longer image dimension, 2 all four straight paths,
+                      // 3 every path; the others run 16-lane ones
 };
 
 inline Schedule default_schedule(int K) { return Schedule{default_prefetch(K), 1, 256, 0}; }

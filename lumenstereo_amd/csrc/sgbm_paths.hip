@@ -40,7 +40,7 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, 
     } else if (j.sched.aggPriority == 2) {
         set_priority_for_length(scan_length(path, s0, j.W, j.H));
     }
-    if constexpr (K == 4 || K == 8 || K == 16 || K == 32) {
+    if constexpr (wide_scanlines_ok(K, PF)) {
         if (wide) {
             paths_wave<K / 2, PAD, PF, 32>(j, path, s0, trash);
             return;
@@ -88,8 +88,9 @@ int wide_path_mask(int wide, int W, int H, int npaths) {
 
 hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s) {
     PathsGrid g;
-    g.wideMask = wide_scanlines_ok(j.Dp) ? wide_path_mask(j.sched.aggWide, j.W, j.H, j.npaths) : 0;
-    if (j.sched.aggWide != 0 && !wide_scanlines_ok(j.Dp)) return hipErrorInvalidValue;
+    if (j.sched.aggWide != 0 && !wide_scanlines_ok(j.K, j.sched.aggPrefetch))
+        return hipErrorInvalidValue;
+    g.wideMask = wide_path_mask(j.sched.aggWide, j.W, j.H, j.npaths);
     int waves = 0;
     for (int r = 0; r < j.npaths; ++r) {
         g.waveStart[r] = waves;

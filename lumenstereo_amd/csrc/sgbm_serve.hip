@@ -1,16 +1,28 @@
 // sgbm_serve.hip - serving form: requests admitted into a device pool by footprint and run in
 // continuous batches over several lanes.
 //
-// One dispatcher thread owns the device side. Each pass it
-//   1. admits waiting requests whose working set fits the pool (serve_core.h Admission),
-//   2. retires lanes whose batch has finished (outputs copied out, pool ranges released),
-//   3. starts admitted requests that have not started on each idle lane, as one batch of
-//      requests of similar size (pick_batch), so small ones never wait for a large one.
-// A lane is a stream with its own BatchEngine (persistent per-stage kernels in one hipGraph), so
-// batches on different lanes overlap on the device, and requests that arrive while a batch runs
-// start on the next free lane instead of waiting for it. Each pair's result is the batched
-// pipeline's, which is bit-identical to the per-pair path.
+// A lane is a stream with its own BatchEngine (persistent per-stage kernels in one hipGraph, or
+// the per-pair kernels for a batch of one), so batches on different lanes overlap on the
+// device, and requests that arrive while a batch runs start on the next free lane instead of
+// waiting for it. Each pair's result is the batched pipeline's, which is bit-identical to the
+// per-pair path.
+//
+// Who does what:
+//   - the submitting thread validates the request, copies host images into a pinned block of
+//     the server's pinned pool, queues the request for admission, and, when a lane is idle and
+//     nothing else is ready or waiting, admits it and starts it on that lane itself (no
+//     hand-over to another thread);
+//   - the dispatcher thread admits waiting requests whose working set fits the device pool
+//     (serve_core.h Admission), retires lanes whose batch has finished (pool ranges released,
+//     requests marked done) and starts ready requests on idle lanes, as one batch of requests
+//     of similar size (pick_batch), so small ones never wait for a large one;
+//   - a completion thread copies finished outputs from their pinned blocks into the callers'
+//     buffers and marks the requests done, in the order their batches finish, so a lane is
+//     free again as soon as its batch is, and no result waits for another one's waiter.
+// A request that found no room in the pinned pool is staged through its lane's own pinned
+// buffers instead, by whichever thread starts and retires its batch.
 #include <hip/hip_runtime.h>
+#include <sys/prctl.h>
 #include <time.h>
 
 #include <algorithm>
@@ -53,18 +65,31 @@ struct HipBacking : PoolBacking {
     void release(void* p) override { (void)hipFree(p); }
 };
 
+struct PinnedBacking : PoolBacking {
+    void* acquire(size_t bytes) override {
+        void* p = nullptr;
+        return hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess ? p : nullptr;
+    }
+    void release(void* p) override { (void)hipHostFree(p); }
+};
+
 struct Request {
     uint64_t id = 0;
     lumen_pair_desc d{};
     size_t bytes = 0;
     double work = 0;  // algorithmic HBM bytes of the pair: W*H*Dp*(1+5*paths)
+    bool large = false;  // above the server's large_us: runs alone on lane 0
     char* region = nullptr;
     int16_t* devOut = nullptr;  // the region's output map (host-memory requests)
-    size_t inOff = 0, outOff = 0;
+    // Pinned staging of a host request, filled at submit: left at 0, right at
+    // align_up(W*H, 256) (the gap of the device region's layout), the output at pinOut.
+    char* pin = nullptr;
+    size_t pinOut = 0;
+    size_t inOff = 0, outOff = 0;  // offsets in the lane's staging (requests without `pin`)
     int status = LUMEN_OK;
     std::string err;
     lumen_request_info info{};
-    bool done = false;
+    bool done = false;     // device work finished and retired (output staged or written)
     bool waiting = false;  // a thread is in lumen_server_wait for this ticket
 };
 
@@ -72,7 +97,8 @@ struct Lane {
     int index = 0;
     BatchEngine engine;
     hipEvent_t fin = nullptr;
-    bool busy = false;
+    bool busy = false;      // claimed for a batch
+    bool launched = false;  // its batch is enqueued and `fin` recorded after it
     std::vector<Request*> reqs;
     std::string err;
     void* pinIn = nullptr;
@@ -98,6 +124,8 @@ hipError_t grow_pinned(void*& p, size_t& cap, size_t need) {
     return e;
 }
 
+size_t staged_input_bytes(size_t n) { return align_up(n, 256) + n; }
+
 }  // namespace
 
 struct lumen_server {
@@ -105,26 +133,43 @@ struct lumen_server {
     int device = 0;
     HipBacking backing;
     DevicePool pool;
+    PinnedBacking pinBacking;
+    DevicePool pinned;  // host staging blocks of host-memory requests
     Admission admission;
     std::vector<std::unique_ptr<Lane>> lanes;
     std::mutex m;
     std::condition_variable wake;    // dispatcher: new work or shutdown
     std::condition_variable doneCv;  // waiters: a request finished
+    std::condition_variable copyCv;  // completion thread: outputs to deliver or shutdown
+    std::deque<Request*> finished;   // device work done, output still in its pinned block
+    bool copyStop = false;
+    std::thread copier;
     std::unordered_map<uint64_t, std::unique_ptr<Request>> reqs;
     std::deque<Request*> ready;      // admitted, not started, in admission order
+    std::deque<Request*> readyLarge; // the same for large requests (lane 0, one at a time)
     uint64_t nextId = 1;
     bool stop = false;
     int busyLanes = 0;
     lumen_server_stats stats{};
     std::thread thread;
 
-    bool idle_locked() const { return admission.empty() && ready.empty() && busyLanes == 0; }
+    bool idle_locked() const {
+        return admission.empty() && ready.empty() && readyLarge.empty() && busyLanes == 0 &&
+               finished.empty() && copying == 0;
+    }
+    int copying = 0;  // outputs the completion thread is copying right now
+    // Whether lane i may start a batch now, and from which queue.
+    std::deque<Request*>* queue_for(size_t i) {
+        if (cfg.large_us > 0 && i == 0 && !readyLarge.empty()) return &readyLarge;
+        return ready.empty() ? nullptr : &ready;
+    }
 };
 
 namespace {
 
 // Enqueues one batch on a lane: uploads (host requests), the pipeline, downloads, the finish
-// event. Runs without the server lock; the lane and its requests belong to the dispatcher.
+// event. Runs without the server lock; a claimed lane and its requests belong to the thread
+// that claimed it until it is marked launched.
 int start_lane(lumen_server* S, Lane& L) {
     const bool hostIo = S->cfg.device_io == 0;
     hipStream_t s = L.engine.stream();
@@ -138,12 +183,14 @@ int start_lane(lumen_server* S, Lane& L) {
         r->devOut = layout_pair(j, r->region);
         const size_t n = (size_t)j.W * j.H;
         if (hostIo) {
-            // left and right are adjacent in the region (each rounded up to 256 bytes), so one
-            // copy uploads both from a staging block with the same gap.
-            r->inOff = inBytes;
-            inBytes += 2 * align_up(n, 256);
-            r->outOff = outBytes;
-            outBytes += align_up(n * 2, 256);
+            if (!r->pin) {
+                // left and right are adjacent in the region (each rounded up to 256 bytes), so
+                // one copy uploads both from a staging block with the same gap.
+                r->inOff = inBytes;
+                inBytes += 2 * align_up(n, 256);
+                r->outOff = outBytes;
+                outBytes += align_up(n * 2, 256);
+            }
         } else {
             j.left = r->d.left;
             j.right = r->d.right;
@@ -159,31 +206,37 @@ int start_lane(lumen_server* S, Lane& L) {
         return hip_code(e);
     };
     if (hostIo) {
-        hipError_t e = grow_pinned(L.pinIn, L.pinInCap, inBytes);
-        if (e == hipSuccess) e = grow_pinned(L.pinOut, L.pinOutCap, outBytes);
-        if (e != hipSuccess) return fail(e, "pinned staging");
-        uint8_t* pin = static_cast<uint8_t*>(L.pinIn);
-        std::vector<CopyJob> stage;
-        for (Request* r : L.reqs) {
-            const size_t n = (size_t)r->d.width * r->d.height;
-            stage.push_back({pin + r->inOff, r->d.left, n});
-            stage.push_back({pin + r->inOff + align_up(n, 256), r->d.right, n});
+        if (inBytes) {
+            hipError_t e = grow_pinned(L.pinIn, L.pinInCap, inBytes);
+            if (e == hipSuccess) e = grow_pinned(L.pinOut, L.pinOutCap, outBytes);
+            if (e != hipSuccess) return fail(e, "pinned staging");
+            uint8_t* pin = static_cast<uint8_t*>(L.pinIn);
+            std::vector<CopyJob> stage;
+            for (Request* r : L.reqs) {
+                if (r->pin) continue;
+                const size_t n = (size_t)r->d.width * r->d.height;
+                stage.push_back({pin + r->inOff, r->d.left, n});
+                stage.push_back({pin + r->inOff + align_up(n, 256), r->d.right, n});
+            }
+            parallel_copy(stage);
         }
-        parallel_copy(stage);
         for (Request* r : L.reqs) {
             const size_t n = (size_t)r->d.width * r->d.height;
-            e = hipMemcpyAsync(r->region, pin + r->inOff, align_up(n, 256) + n,
-                               hipMemcpyHostToDevice, s);
+            const void* src = r->pin ? static_cast<const void*>(r->pin)
+                                     : static_cast<uint8_t*>(L.pinIn) + r->inOff;
+            hipError_t e = hipMemcpyAsync(r->region, src, staged_input_bytes(n),
+                                          hipMemcpyHostToDevice, s);
             if (e != hipSuccess) return fail(e, "upload");
         }
     }
     int rc = L.engine.enqueue(pl, L.err);
     if (rc) return rc;
     if (hostIo) {
-        uint8_t* pout = static_cast<uint8_t*>(L.pinOut);
         for (Request* r : L.reqs) {
             const size_t n = (size_t)r->d.width * r->d.height;
-            hipError_t e = hipMemcpyAsync(pout + r->outOff, r->devOut, n * 2, hipMemcpyDeviceToHost, s);
+            void* dst = r->pin ? static_cast<void*>(r->pin + r->pinOut)
+                               : static_cast<uint8_t*>(L.pinOut) + r->outOff;
+            hipError_t e = hipMemcpyAsync(dst, r->devOut, n * 2, hipMemcpyDeviceToHost, s);
             if (e != hipSuccess) return fail(e, "download");
         }
     }
@@ -192,7 +245,8 @@ int start_lane(lumen_server* S, Lane& L) {
     return LUMEN_OK;
 }
 
-// Copies a finished lane's outputs to the callers (host requests). No server lock needed.
+// Copies a finished lane's outputs of requests without a pinned block to their callers (the
+// others are copied by their waiters). No server lock needed.
 void finish_lane(lumen_server* S, Lane& L, hipError_t status) {
     if (status != hipSuccess) {
         for (Request* r : L.reqs) {
@@ -205,47 +259,130 @@ void finish_lane(lumen_server* S, Lane& L, hipError_t status) {
         const uint8_t* pout = static_cast<const uint8_t*>(L.pinOut);
         std::vector<CopyJob> outs;
         for (Request* r : L.reqs)
-            outs.push_back({r->d.out, pout + r->outOff, (size_t)r->d.width * r->d.height * 2});
+            if (!r->pin)
+                outs.push_back({r->d.out, pout + r->outOff, (size_t)r->d.width * r->d.height * 2});
         parallel_copy(outs);
     }
 }
 
+void complete_locked(lumen_server* S, Request* r, int64_t t) {
+    r->info.status = r->status;
+    r->info.t_done_ns = t;
+    r->done = true;
+    if (r->status == LUMEN_OK)
+        ++S->stats.completed;
+    else
+        ++S->stats.failed;
+}
+
+// Frees a finished lane and its requests' device ranges. Requests whose output waits in a
+// pinned block go to the completion thread; the others are complete.
 void retire_locked(lumen_server* S, Lane& L) {
     const int64_t t = monotonic_ns();
+    bool toCopy = false;
     for (Request* r : L.reqs) {
         S->pool.release(r->region);
         r->region = nullptr;
-        r->info.status = r->status;
-        r->info.t_done_ns = t;
-        r->done = true;
-        if (r->status == LUMEN_OK)
-            ++S->stats.completed;
-        else
-            ++S->stats.failed;
+        if (r->pin && r->status == LUMEN_OK) {
+            S->finished.push_back(r);
+            toCopy = true;
+        } else {
+            complete_locked(S, r, t);
+        }
     }
     L.reqs.clear();
     L.busy = false;
+    L.launched = false;
     --S->busyLanes;
+    if (toCopy) S->copyCv.notify_one();
+}
+
+// Completion thread: delivers staged outputs to the callers' buffers, then completes them.
+void copier(lumen_server* S) {
+    std::unique_lock<std::mutex> lk(S->m);
+    for (;;) {
+        S->copyCv.wait(lk, [&] { return S->copyStop || !S->finished.empty(); });
+        if (S->finished.empty()) return;  // stopping, nothing left
+        Request* r = S->finished.front();
+        S->finished.pop_front();
+        ++S->copying;
+        lk.unlock();
+        parallel_copy({{r->d.out, r->pin + r->pinOut, (size_t)r->d.width * r->d.height * 2}});
+        lk.lock();
+        --S->copying;
+        S->pinned.release(r->pin);
+        r->pin = nullptr;
+        complete_locked(S, r, monotonic_ns());
+        S->doneCv.notify_all();
+    }
+}
+
+// Hands out pool ranges to waiting requests that fit; they join a ready queue.
+void admit_locked(lumen_server* S) {
+    for (const Admission::Admitted& a : S->admission.admit()) {
+        Request* r = S->reqs[a.id].get();
+        r->region = static_cast<char*>(a.region);
+        r->info.t_admit_ns = monotonic_ns();
+        (r->large ? S->readyLarge : S->ready).push_back(r);
+    }
+    S->stats.pool_high_water = S->pool.high_water();
+}
+
+// Moves the next batch of queue q onto idle lane L and claims it (a large request runs alone).
+void claim_locked(lumen_server* S, Lane& L, std::deque<Request*>& q) {
+    const size_t maxBatch = &q == &S->readyLarge ? 1 : static_cast<size_t>(S->cfg.max_batch);
+    const double budget = S->cfg.batch_budget_us * 5e6;  // bytes at 5 TB/s
+    std::vector<double> work;
+    for (const Request* r : q) work.push_back(r->work);
+    const std::vector<size_t> pick = pick_batch(work, maxBatch, budget, 4.0);
+    for (size_t k = pick.size(); k-- > 0;) {
+        L.reqs.push_back(q[pick[k]]);
+        q.erase(q.begin() + static_cast<std::ptrdiff_t>(pick[k]));
+    }
+    std::reverse(L.reqs.begin(), L.reqs.end());
+    L.busy = true;
+    L.launched = false;
+    ++S->busyLanes;
+    ++S->stats.batches;
+    S->stats.max_batch_seen = std::max<uint64_t>(S->stats.max_batch_seen, L.reqs.size());
+    S->stats.max_lanes_busy = std::max<uint64_t>(S->stats.max_lanes_busy, S->busyLanes);
+}
+
+// Enqueues a claimed lane's batch; called with the lock held through `lk`, which is released
+// while the copies and launches are enqueued.
+void launch_claimed(lumen_server* S, Lane& L, std::unique_lock<std::mutex>& lk) {
+    lk.unlock();
+    const int rc = start_lane(S, L);
+    if (rc) {
+        // Nothing of a failed start is left in flight once the lane's stream drains.
+        (void)hipStreamSynchronize(L.engine.stream());
+        for (Request* r : L.reqs) {
+            r->status = rc;
+            r->err = L.err;
+        }
+    }
+    lk.lock();
+    if (rc) {
+        retire_locked(S, L);
+        S->doneCv.notify_all();
+    } else {
+        L.launched = true;
+    }
 }
 
 void dispatcher(lumen_server* S) {
     (void)hipSetDevice(S->device);
-    const size_t maxBatch = static_cast<size_t>(S->cfg.max_batch);
-    const double budget = S->cfg.batch_budget_us * 5e6;  // bytes at 5 TB/s
-    std::vector<double> work;
+    // Timed waits of a few microseconds: the default 50 us timer slack would stretch each poll
+    // of the lanes several times over.
+    (void)prctl(PR_SET_TIMERSLACK, 1UL, 0UL, 0UL, 0UL);
     std::unique_lock<std::mutex> lk(S->m);
     for (;;) {
         bool progress = false;
-        for (const Admission::Admitted& a : S->admission.admit()) {
-            Request* r = S->reqs[a.id].get();
-            r->region = static_cast<char*>(a.region);
-            r->info.t_admit_ns = monotonic_ns();
-            S->ready.push_back(r);
-            progress = true;
-        }
-        S->stats.pool_high_water = S->pool.high_water();
+        const size_t readyBefore = S->ready.size();
+        admit_locked(S);
+        progress |= S->ready.size() != readyBefore;
         for (auto& L : S->lanes) {
-            if (!L->busy) continue;
+            if (!L->launched) continue;
             const hipError_t q = hipEventQuery(L->fin);
             if (q == hipErrorNotReady) continue;
             lk.unlock();
@@ -255,43 +392,20 @@ void dispatcher(lumen_server* S) {
             S->doneCv.notify_all();
             progress = true;
         }
-        for (auto& L : S->lanes) {
-            if (L->busy || S->ready.empty()) continue;
-            work.clear();
-            for (const Request* r : S->ready) work.push_back(r->work);
-            const std::vector<size_t> pick = pick_batch(work, maxBatch, budget, 4.0);
-            for (size_t k = pick.size(); k-- > 0;) {
-                L->reqs.push_back(S->ready[pick[k]]);
-                S->ready.erase(S->ready.begin() + static_cast<std::ptrdiff_t>(pick[k]));
-            }
-            std::reverse(L->reqs.begin(), L->reqs.end());
-            L->busy = true;
-            ++S->busyLanes;
-            ++S->stats.batches;
-            S->stats.max_batch_seen = std::max<uint64_t>(S->stats.max_batch_seen, L->reqs.size());
-            S->stats.max_lanes_busy = std::max<uint64_t>(S->stats.max_lanes_busy, S->busyLanes);
-            lk.unlock();
-            const int rc = start_lane(S, *L);
-            if (rc) {
-                // Nothing of a failed start is left in flight once the lane's stream drains.
-                (void)hipStreamSynchronize(L->engine.stream());
-                for (Request* r : L->reqs) {
-                    r->status = rc;
-                    r->err = L->err;
-                }
-            }
-            lk.lock();
-            if (rc) {
-                retire_locked(S, *L);
-                S->doneCv.notify_all();
-            }
+        if (progress) admit_locked(S);  // retired ranges may let waiting requests in
+        for (size_t i = 0; i < S->lanes.size(); ++i) {
+            Lane& L = *S->lanes[i];
+            std::deque<Request*>* q = S->queue_for(i);
+            if (L.busy || !q) continue;
+            claim_locked(S, L, *q);
+            launch_claimed(S, L, lk);
             progress = true;
         }
         if (S->stop && S->idle_locked()) break;
         if (!progress) {
             // Poll the lanes often while batches run; otherwise sleep until a submit.
             if (S->busyLanes > 0)
-                S->wake.wait_for(lk, std::chrono::microseconds(20));
+                S->wake.wait_for(lk, std::chrono::microseconds(10));
             else
                 S->wake.wait_for(lk, std::chrono::milliseconds(5));
         }
@@ -316,8 +430,9 @@ int lumen_server_create(const lumen_server_config* cfg, lumen_server** out) {
     if (c.max_batch == 0) c.max_batch = 64;
     if (c.max_bypass == 0) c.max_bypass = 8;
     if (c.batch_budget_us <= 0) c.batch_budget_us = 2000;
-    if (c.lanes < 1 || c.lanes > 4 || c.max_batch < 1 || c.max_batch > 65535) {
-        g_serve_error = "lanes must be 1..4 and max_batch 1..65535";
+    if (c.lanes < 1 || c.lanes > 4 || c.max_batch < 1 || c.max_batch > 65535 ||
+        c.pinned_mb < 0 || c.large_us < 0) {
+        g_serve_error = "lanes must be 1..4, max_batch 1..65535, pinned_mb and large_us >= 0";
         return LUMEN_EINVAL;
     }
     std::unique_ptr<lumen_server> S(new (std::nothrow) lumen_server());
@@ -343,6 +458,11 @@ int lumen_server_create(const lumen_server_config* cfg, lumen_server** out) {
     }
     S->admission = Admission(&S->pool, c.max_bypass < 0 ? 0 : c.max_bypass);
     S->stats.pool_bytes = S->pool.capacity();
+    if (c.device_io == 0) {
+        // Without a pinned pool every request is staged through its lane's buffers instead.
+        const size_t mb = c.pinned_mb > 0 ? static_cast<size_t>(c.pinned_mb) : 1024;
+        (void)S->pinned.open(&S->pinBacking, mb << 20);
+    }
     for (int i = 0; i < c.lanes; ++i) {
         std::unique_ptr<Lane> L(new Lane());
         L->index = i;
@@ -356,9 +476,18 @@ int lumen_server_create(const lumen_server_config* cfg, lumen_server** out) {
     }
     lumen_server* raw = S.get();
     try {
+        S->copier = std::thread(copier, raw);
         S->thread = std::thread(dispatcher, raw);
     } catch (...) {
-        g_serve_error = "lumen_server_create: cannot start the dispatcher thread";
+        if (S->copier.joinable()) {
+            {
+                std::lock_guard<std::mutex> g(S->m);
+                S->copyStop = true;
+            }
+            S->copyCv.notify_all();
+            S->copier.join();
+        }
+        g_serve_error = "lumen_server_create: cannot start the server threads";
         return LUMEN_ENOMEM;
     }
     *out = S.release();
@@ -372,7 +501,13 @@ int lumen_server_destroy(lumen_server* s) {
         s->stop = true;
     }
     s->wake.notify_all();
-    if (s->thread.joinable()) s->thread.join();
+    if (s->thread.joinable()) s->thread.join();  // returns once nothing is queued or copying
+    {
+        std::lock_guard<std::mutex> g(s->m);
+        s->copyStop = true;
+    }
+    s->copyCv.notify_all();
+    if (s->copier.joinable()) s->copier.join();
     delete s;
     return LUMEN_OK;
 }
@@ -408,22 +543,51 @@ int lumen_server_submit(lumen_server* s, const lumen_pair_desc* pair, uint64_t* 
     r->work = static_cast<double>(d.width) * d.height *
               padded_range(d.params.maxDisparity - d.params.minDisparity) *
               (1 + 5 * num_paths(d.params));
+    r->large = s->cfg.large_us > 0 && r->work > s->cfg.large_us * 5e6;
     r->info.t_submit_ns = monotonic_ns();
     r->info.lane = -1;
-    {
-        std::lock_guard<std::mutex> g(s->m);
-        if (s->stop) {
-            g_serve_error = "server is shutting down";
-            return LUMEN_EINVAL;
+    const bool hostIo = s->cfg.device_io == 0;
+    const size_t n = (size_t)d.width * d.height;
+    if (hostIo) {
+        // Stage the images now, in this thread: the dispatcher then only enqueues copies.
+        r->pinOut = align_up(staged_input_bytes(n), 256);
+        {
+            std::lock_guard<std::mutex> g(s->m);
+            r->pin = static_cast<char*>(s->pinned.alloc(r->pinOut + 2 * n));
+            if (!r->pin) ++s->stats.unstaged;
         }
-        const uint64_t id = s->nextId++;
-        r->id = id;
-        *ticket = id;
-        s->admission.push(id, bytes);
-        s->reqs[id] = std::move(r);
-        ++s->stats.submitted;
-        s->stats.max_waiting = std::max<uint64_t>(s->stats.max_waiting, s->admission.waiting());
+        if (r->pin)
+            parallel_copy({{r->pin, d.left, n}, {r->pin + align_up(n, 256), d.right, n}});
     }
+    Lane* direct = nullptr;
+    std::unique_lock<std::mutex> lk(s->m);
+    if (s->stop) {
+        if (r->pin) s->pinned.release(r->pin);
+        g_serve_error = "server is shutting down";
+        return LUMEN_EINVAL;
+    }
+    const uint64_t id = s->nextId++;
+    r->id = id;
+    *ticket = id;
+    s->admission.push(id, bytes);
+    s->reqs[id] = std::move(r);
+    ++s->stats.submitted;
+    s->stats.max_waiting = std::max<uint64_t>(s->stats.max_waiting, s->admission.waiting());
+    // Nothing ahead of it and a lane idle: admit and start it from here, without waking the
+    // dispatcher first.
+    if (s->ready.empty() && s->readyLarge.empty() && s->admission.waiting() == 1) {
+        admit_locked(s);
+        for (size_t i = 0; i < s->lanes.size(); ++i) {
+            std::deque<Request*>* q = s->queue_for(i);
+            if (s->lanes[i]->busy || !q) continue;
+            direct = s->lanes[i].get();
+            claim_locked(s, *direct, *q);
+            ++s->stats.direct_starts;
+            break;
+        }
+    }
+    if (direct) launch_claimed(s, *direct, lk);
+    lk.unlock();
     s->wake.notify_one();
     return LUMEN_OK;
 }
@@ -455,6 +619,10 @@ int lumen_server_wait(lumen_server* s, uint64_t ticket, int64_t timeout_us,
         r->waiting = false;
         g_serve_error = "timed out";
         return LUMEN_ETIMEDOUT;
+    }
+    if (r->pin) {  // a failed request's staging block (delivered ones are released already)
+        s->pinned.release(r->pin);
+        r->pin = nullptr;
     }
     if (info) *info = r->info;
     const int status = r->status;

@@ -141,9 +141,10 @@ bool valid_schedule(const Schedule& s, int K, std::string& why) {
         why = "aggregation lane setting must be 0, 1, 2 or 3";
         return false;
     }
-    if (s.aggWide != 0 && !wide_scanlines_ok(K * kScanLanes)) {
-        why = "32-lane scanlines need 64, 128, 256 or 512 padded disparities, not " +
-              std::to_string(K * kScanLanes);
+    if (s.aggWide != 0 && !wide_scanlines_ok(K, s.aggPrefetch)) {
+        why = "aggregation prefetch " + std::to_string(s.aggPrefetch) +
+              " is not supported on 32-lane scanlines for " + std::to_string(K * kScanLanes) +
+              " padded disparities";
         return false;
     }
     return true;

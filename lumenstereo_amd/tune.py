@@ -10,7 +10,7 @@ and ordered on the device, never what is computed:
 * ``agg_wide``: which aggregation paths run scanlines of 32 lanes (two per wave, half the
   disparities per lane: fewer instructions on each step's dependency chain, twice the waves)
   instead of 16 (four per wave): none, the straight pair along the longer image dimension, all
-  straight paths, or every path (padded ranges 64, 128, 256 and 512 only).
+  straight paths, or every path.
 
 For each tuning key (width bucket, padded disparity range Dp, cost mode, path count) the tuner
 times the candidates on the GPU. It is coordinate descent: first the lanes per scanline, then
@@ -51,8 +51,10 @@ def valid_prefetch(K: int, pf: int) -> bool:
 
 
 def wide_settings(K: int):
-    """agg_wide values a slice of K disparities per 16 lanes supports (wide_scanlines_ok)."""
-    return (0, 1, 2, 3) if 16 * K in (64, 128, 256, 512) else (0,)
+    """agg_wide values a slice of K disparities per 16 lanes supports: every one (32-lane
+    scanlines split any padded range into whole chunks, and store no more per step than
+    16-lane ones, so every valid prefetch depth fits them too)."""
+    return (0, 1, 2, 3)
 
 
 def default_schedule(K: int) -> dict:

@@ -155,23 +155,25 @@ def test_submit_and_wait_from_several_threads(gpu_available):
 
 def test_wait_timeout_keeps_the_request(gpu_available):
     """A wait that times out leaves the request queued or running with its buffers alive; a
-    later wait returns its result, equal to the per-pair path's. A zero timeout on a 1080p
-    request submitted behind three others times out (it cannot be done in no time)."""
-    p = ls.StereoParams(maxDisparity=128, numDirections=8,
+    later wait returns its result, equal to the per-pair path's. Six 2880x1988 D=128 requests
+    run one after another on one lane (about 4 ms each), and each is waited on first with a zero
+    timeout right after the previous one returned: the second one cannot be done then."""
+    p = ls.StereoParams(maxDisparity=128, numDirections=4,
                         matchingCostMode=ls.MatchingCostMode.Census, uniquenessRatio=0,
                         speckleWindowSize=0)
-    l, r = ls.synth_pair(1920, 1080, 4, 120, seed=31)
+    l, r = ls.synth_pair(2880, 1988, 4, 120, seed=31)
     want = ls.StereoSGBM(p).compute(l, r)
-    srv = ls.ServeSGBM(pool_bytes=24 << 30, lanes=1, max_batch=1)
-    tickets = [srv.submit(p, l, r) for _ in range(4)]
-    timed_out = 0
+    srv = ls.ServeSGBM(pool_bytes=16 << 30, lanes=1, max_batch=1)
+    tickets = [srv.submit(p, l, r) for _ in range(6)]
+    results, timed_out = [], 0
     for t in tickets:
         try:
-            out, info = srv.wait(t, timeout_s=0)
+            results.append(srv.wait(t, timeout_s=0))
         except TimeoutError:
             timed_out += 1
-            out, info = srv.wait(t)
+            results.append(srv.wait(t))
+    srv.close()
+    for out, info in results:
         assert info["status"] == 0
         np.testing.assert_array_equal(out, want)
-    srv.close()
     assert timed_out >= 1

@@ -12,15 +12,16 @@ from lumenstereo_amd import tune
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    # (W, H, D, census, paths)
-    (96, 64, 64, False, 4),     # K = 4 (32-lane scanlines: 2 per lane)
-    (130, 70, 128, True, 8),    # K = 8 (32-lane: 4 per lane)
-    (120, 60, 170, True, 4),    # K = 12, padded
-    (150, 50, 290, False, 4),   # K = 20, padded
-    (100, 40, 384, True, 8),    # K = 24
+    # (W, H, D, census, paths); 32-lane scanlines hold K/2 disparities per lane
+    (96, 64, 64, False, 4),     # K = 4 (32-lane: one chunk of 2)
+    (130, 70, 128, True, 8),    # K = 8 (32-lane: one chunk of 4)
+    (120, 60, 170, True, 4),    # K = 12, padded (32-lane: three chunks of 2)
+    (150, 50, 290, False, 4),   # K = 20, padded (32-lane: five chunks of 2)
+    (100, 40, 384, True, 8),    # K = 24 (32-lane: three chunks of 4)
     (60, 90, 40, True, 8),      # K = 4 padded (40 -> 64), taller than wide
-    (100, 33, 250, False, 8),   # K = 16 padded (32-lane: 8 per lane)
-    (70, 41, 512, True, 4),     # K = 32 (32-lane: 16 per lane)
+    (100, 33, 250, False, 8),   # K = 16 padded (32-lane: one chunk of 8)
+    (70, 41, 512, True, 4),     # K = 32 (32-lane: one chunk of 16)
+    (80, 45, 448, False, 8),    # K = 28 (32-lane: seven chunks of 2)
 ]
 
 
@@ -49,7 +50,10 @@ def test_every_candidate_schedule_is_bit_identical(gpu_available, case):
 @pytest.mark.parametrize("case", [(96, 64, 0, 64, 0, 4, {}), (130, 77, 0, 40, 1, 8, {}),
                                   (64, 150, 3, 131, 0, 8, {}), (100, 70, 0, 32, 1, 8,
                                                                  {"P1": 30000, "P2": 60000}),
-                                  (90, 50, 0, 256, 1, 4, {}), (80, 30, 0, 512, 0, 8, {})],
+                                  (90, 50, 0, 256, 1, 4, {}), (80, 30, 0, 512, 0, 8, {}),
+                                  (90, 50, 0, 192, 1, 8, {}), (70, 40, 5, 295, 0, 4, {}),
+                                  (60, 30, 0, 448, 1, 8, {}), (66, 35, 0, 384, 0, 4,
+                                                               {"P1": 30000, "P2": 60000})],
                          ids=lambda c: f"{c[0]}x{c[1]}_d{c[2]}-{c[3]}_m{c[4]}_p{c[5]}")
 def test_wide_scanlines_every_path_equals_oracle(gpu_available, case):
     """Every path on 32-lane scanlines: each path's aggregated costs equal the oracle's (wide
@@ -83,9 +87,8 @@ def test_unsupported_schedule_is_refused(gpu_available):
     m.setSchedule({"agg_prefetch": 8, "agg_priority": 1, "census_tile": 100})
     with pytest.raises(ls.StereoException, match="census tile"):
         m.compute(left, right)
-    # 320 padded disparities cannot be split over 32 lanes in whole chunks.
-    m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 1})
-    with pytest.raises(ls.StereoException, match="32-lane"):
+    m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 4})
+    with pytest.raises(ls.StereoException, match="lane setting"):
         m.compute(left, right)
     m.setSchedule(None)
     np.testing.assert_array_equal(m.compute(left, right), oracle.sgbm(p, left, right))

@@ -61,7 +61,6 @@ def test_override_beats_table(tmp_path):
     ("640 128 1 8 16 2 256 1 5", "expected 7 or 8 integers"),
     ("640 128 1 8 16 2 256 x", "expected 7 or 8 integers"),
     ("640 128 1 8 16 2 256 4", "lane setting"),
-    ("640 192 1 8 8 1 256 1", "32-lane scanlines need"),
     ("640 100 1 8 8 1 256", "bad width or padded range"),
     ("640 128 2 8 8 1 256", "bad cost mode"),
     ("640 128 1 6 8 1 256", "bad cost mode or path count"),
@@ -101,19 +100,21 @@ def test_missing_file_is_an_error():
 
 
 def test_lane_setting_column(tmp_path):
-    """The optional eighth column is the aggregation lane setting (0 when absent); ranges that
-    32 lanes cannot split into whole chunks accept only 0."""
+    """The optional eighth column is the aggregation lane setting (0 when absent); every padded
+    range takes every setting with every prefetch depth its 16-lane scanlines take."""
     f = tmp_path / "t.tsv"
-    f.write_text("640 128 1 8 16 2 128 3\n640 64 1 8 16 2 128\n640 192 1 8 8 1 256 0\n")
+    f.write_text("640 128 1 8 16 2 128 3\n640 64 1 8 16 2 128\n640 192 1 8 8 1 256 2\n")
     assert ls.tuning_load(str(f)) == 3
     assert _handle(128, True, 8).getSchedule(640, 480)["agg_wide"] == 3
     assert _handle(64, True, 8).getSchedule(640, 480)["agg_wide"] == 0
+    assert _handle(192, True, 8).getSchedule(640, 480)["agg_wide"] == 2
     for Dp in range(64, 513, 64):
-        g = tmp_path / f"w{Dp}.tsv"
-        g.write_text(f"640 {Dp} 1 4 4 1 256 1\n")
-        try:
-            ls.tuning_load(str(g))
-            ok = True
-        except ls.StereoException:
-            ok = False
-        assert ok == (tune.wide_settings(Dp // 16) != (0,)), Dp
+        for pf in (4, 8, 16):
+            g = tmp_path / f"w{Dp}_{pf}.tsv"
+            g.write_text(f"640 {Dp} 1 4 {pf} 1 256 3\n")
+            try:
+                ls.tuning_load(str(g))
+                ok = True
+            except ls.StereoException:
+                ok = False
+            assert ok == tune.valid_prefetch(Dp // 16, pf), (Dp, pf)
