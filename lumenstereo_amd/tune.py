@@ -13,9 +13,9 @@ and ordered on the device, never what is computed:
   straight paths, or every path.
 
 For each tuning key (width bucket, padded disparity range Dp, cost mode, path count) the tuner
-times the candidates on the GPU. It is coordinate descent: first the lanes per scanline, then
-prefetch and priority, on the aggregation stage time; then the census tile on the cost stage
-time. The candidates of a step
+times the candidates on the GPU. It is coordinate descent: first the lanes per scanline and the
+prefetch depth together (they interact: wider scanlines can need deeper prefetch), then the
+priority, on the aggregation stage time; then the census tile on the cost stage time. The candidates of a step
 are timed round-robin over several rounds and each is scored by its median round, so a slow
 drift of the device's speed (which is as large as the differences being measured, a few per
 cent) falls on every candidate alike instead of favouring whichever was timed last. It checks
@@ -145,15 +145,16 @@ def tune(keys, reps_small: int = 10, rounds: int = 5, log=print):
         reps = reps_small if W * H * Dp < 400e6 else 4
         m = StereoSGBM(p)
         base = default_schedule(K)
-        wcands = [dict(base, agg_wide=w) for w in wide_settings(K)]
-        wtimes, wouts = timed_rounds(m, d_in, (H, W), reps, wcands, 2,
-                                     rounds if len(wcands) > 1 else 1)
+        pfs = [pf for pf in (4, 8, 16) if valid_prefetch(K, pf)]
+        wcands = [dict(base)] + [dict(base, agg_wide=w, agg_prefetch=pf)
+                                 for w in wide_settings(K) for pf in pfs
+                                 if (w, pf) != (0, base["agg_prefetch"])]
+        wtimes, wouts = timed_rounds(m, d_in, (H, W), reps, wcands, 2, rounds)
         ref = wouts[0]
         mismatches = sum(int(not np.array_equal(o, ref)) for o in wouts)
         wb = wcands[pick(wtimes, 0)]
-        cands = [dict(wb)] + [dict(wb, agg_prefetch=pf, agg_priority=prio)
-                              for pf, prio in agg_candidates(K)
-                              if (pf, prio) != (base["agg_prefetch"], base["agg_priority"])]
+        cands = [dict(wb)] + [dict(wb, agg_priority=prio) for prio in (0, 1, 2)
+                              if prio != wb["agg_priority"]]
         times, outs = timed_rounds(m, d_in, (H, W), reps, cands, 2, rounds)
         mismatches += sum(int(not np.array_equal(o, ref)) for o in outs)
         b = pick(times, 0)
