@@ -408,10 +408,11 @@ def cpu_threads_available():
         return os.cpu_count() or 1
 
 
-def bench_single(ls, torch, name, steps, warmup, cpu_threads, cpu_runs, parity=True):
+def bench_single(ls, torch, name, steps, warmup, cpu_threads, cpu_runs, parity=True, timing=2):
     """One single-pair configuration of workloads.SINGLE: `steps` computes from device images
     (inputs resident in HBM) timed by the host clock between two device synchronisations, the
-    stage times from hipEvents on the handle's stream (mean over the timed computes), the
+    stage times from hipEvents on the handle's stream (mean over the timed computes, or over a
+    second pass when `timing` < 2 left the stage events off in the timed one), the
     aggregation's roofline fraction, and optionally the CPU baseline and the pixel-wise
     comparison of the GPU output with the oracle's on the same frame."""
     from lumenstereo_amd import workloads
@@ -424,14 +425,24 @@ def bench_single(ls, torch, name, steps, warmup, cpu_threads, cpu_runs, parity=T
     d_out = torch.empty((H, W), dtype=torch.int16, device=dev)
     m = ls.StereoSGBM(params)
     npaths = m.numPaths()
-    for _ in range(warmup):
-        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
+    m.setTiming(timing)
+
+    def run(n):
+        for _ in range(n):
+            m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(),
+                             sync=False)
+
+    run(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        m.compute_device(d_left.data_ptr(), d_right.data_ptr(), W, H, d_out.data_ptr(), sync=False)
+    run(steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    if timing < 2:
+        # Stage times from a second pass that records them (the timed pass did not).
+        m.setTiming(2)
+        run(min(steps, 64))
+        torch.cuda.synchronize()
     stage = m.timings(ncalls=min(steps, 64))  # total, cost, agg, post, wta, upload
     de = W * H * D
     ms = elapsed * 1e3 / steps
@@ -454,6 +465,9 @@ def bench_single(ls, torch, name, steps, warmup, cpu_threads, cpu_runs, parity=T
         "aggregation_roofline_frac": round(agg_bytes / (agg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "frame_roofline_frac": round(frame_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "schedule": m.getSchedule(W, H),
+        "stage_timing": "stage events recorded in the timed computes" if timing >= 2 else
+                        "timed computes record no stage events (lumen_sgbm_set_timing "
+                        f"{timing}); stage_ms from a second pass that records them",
         "_agg_bytes": agg_bytes,
     }
     gpu_out = d_out.cpu().numpy()
@@ -624,7 +638,8 @@ def main() -> int:
             if name == args.workload:
                 continue
             print(f"# config {name}", file=sys.stderr, flush=True)
-            c = bench_single(ls, torch, name, 20, 3, 0, 0, parity=not args.no_cpu_baseline)
+            c = bench_single(ls, torch, name, 50, 5, 0, 0, parity=not args.no_cpu_baseline,
+                             timing=0)
             c.pop("_agg_bytes")
             line["configs"][name] = c
     if not args.skip_scene:

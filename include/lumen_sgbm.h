@@ -111,14 +111,28 @@ int lumen_sgbm_get_stream(lumen_sgbm* h, void** stream_out);
 
 /* Stage times of the last completed compute in milliseconds, from hipEvents on the handle's
  * stream: out[0] total (as getLastComputeTimeMs, src/sgbm_gpu.cu:177-191: upload through
- * post-processing), out[1] cost (getCostComputeTimeMs), out[2] aggregation
- * (getAggregationTimeMs), out[3] post-processing (getPostProcessTimeMs: speckle and median),
- * out[4] path sum + WTA + LR check, out[5] upload. Waits for the last compute. */
+ * post-processing; for lumen_sgbm_compute_device from the first kernel's start), out[1] cost
+ * (getCostComputeTimeMs), out[2] aggregation (getAggregationTimeMs), out[3] post-processing
+ * (getPostProcessTimeMs: speckle and median), out[4] path sum + WTA + LR check, out[5] upload.
+ * The events are carried by the stage kernels' dispatches (hipExtLaunchKernel), so the stage
+ * times are the kernels' own durations; recording them still costs time (lumen_sgbm_set_timing
+ * turns them off). Waits for the last compute. */
 int lumen_sgbm_get_timings(lumen_sgbm* h, float out[6]);
 
 /* Average of the same six stage times over the last `ncalls` completed computes (at most 64
  * are kept). Waits for them. */
 int lumen_sgbm_get_timings_avg(lumen_sgbm* h, int ncalls, float out[6]);
+
+/* What later computes on the handle time (no reference counterpart: the reference always times
+ * every stage). LUMEN_TIMING_STAGES (the default) records every stage time above;
+ * LUMEN_TIMING_TOTAL only out[0] and out[5] (getLastComputeTimeMs), the others read 0;
+ * LUMEN_TIMING_NONE nothing (all read 0). Each stage event costs host time per call and stream
+ * time between the kernels (about 20 us of a 640x480 D=64 frame), so callers that do not read
+ * the stage times can turn them off. */
+#define LUMEN_TIMING_NONE 0
+#define LUMEN_TIMING_TOTAL 1
+#define LUMEN_TIMING_STAGES 2
+int lumen_sgbm_set_timing(lumen_sgbm* h, int level);
 
 /* Text of the last error on this handle ("" if none). Valid until the next call on it. */
 const char* lumen_sgbm_last_error(const lumen_sgbm* h);

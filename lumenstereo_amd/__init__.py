@@ -235,6 +235,7 @@ _SIGNATURES = {
     "lumen_sgbm_get_stream": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
     "lumen_sgbm_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "lumen_sgbm_get_timings_avg": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(ctypes.c_float)]),
+    "lumen_sgbm_set_timing": (_c_int, [_c_void_p, _c_int]),
     "lumen_sgbm_last_error": (ctypes.c_char_p, [_c_void_p]),
     "lumen_last_error": (ctypes.c_char_p, []),
     "lumen_sgbm_dump_stage": (_c_int, [_c_void_p, _c_int, _c_int, _c_void_p, _c_size_t]),
@@ -431,6 +432,13 @@ class StereoSGBM:
         if rc != LUMEN_OK:
             self._raise(rc)
         return t
+
+    def setTiming(self, level: int) -> None:
+        """What later computes time: 2 every stage (the default), 1 only the total
+        (getLastComputeTimeMs), 0 nothing. Stage events cost host and stream time."""
+        rc = self._lib.lumen_sgbm_set_timing(self._h, int(level))
+        if rc != LUMEN_OK:
+            self._raise(rc)
 
     def dump(self, stage: str, path: int = 0) -> np.ndarray:
         """One stage of the last compute in the reference's layout (parity hook)."""

@@ -85,7 +85,9 @@ struct PinnedBuf {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
-constexpr int kNumEvents = 6;  // start, uploaded, cost, paths, wta, post
+// Per compute: 0 before the upload, then each stage's first kernel start and last kernel end:
+// 1-2 cost, 3-4 aggregation, 5-6 sum + WTA + LR check, 7-8 post-processing.
+constexpr int kNumEvents = 9;
 constexpr int kRing = 64;
 
 struct Geometry {
@@ -104,6 +106,9 @@ struct lumen_sgbm {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[kRing][kNumEvents] = {};
+    bool slotHasUpload[kRing] = {};  // the slot's compute uploaded host images (event 0 used)
+    int slotTiming[kRing] = {};      // the timing level the slot's compute recorded
+    int timing = LUMEN_TIMING_STAGES;
     int ringNext = 0;   // next slot to record
     int ringCount = 0;  // completed-or-pending calls recorded
     Geometry geo;
@@ -225,28 +230,31 @@ PairJob make_job(const lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int1
 }
 
 // Enqueues every stage after the upload; images are already in h->left / h->right or in the
-// caller's device buffers.
+// caller's device buffers. The stage times come from events the stage kernels carry
+// (StageEvents), so they are the kernels' own durations and nothing is enqueued between them.
 int run_stages(lumen_sgbm* h, const uint8_t* dl, const uint8_t* dr, int16_t* dout, int slot) {
     hipStream_t s = h->stream;
     const PairJob j = make_job(h, dl, dr, dout);
-    hipError_t e = launch_cost(j, s);
+    // Each event costs host time and stream time: only those of the handle's timing level.
+    hipEvent_t ev[kNumEvents] = {};
+    h->slotTiming[slot] = h->timing;
+    if (h->timing >= LUMEN_TIMING_TOTAL) {
+        ev[1] = h->ev[slot][1];
+        ev[8] = h->ev[slot][8];
+    }
+    if (h->timing >= LUMEN_TIMING_STAGES)
+        for (int k = 2; k < 8; ++k) ev[k] = h->ev[slot][k];
+    hipError_t e = launch_cost(j, s, {ev[1], ev[2]});
     if (e != hipSuccess) return h->hip_fail(e, "cost launch");
-    e = hipEventRecord(h->ev[slot][2], s);
-    if (e != hipSuccess) return h->hip_fail(e, "event record");
-    e = launch_paths(j, h->trash.as<uint16_t>(), s);
+    e = launch_paths(j, h->trash.as<uint16_t>(), s, {ev[3], ev[4]});
     if (e != hipSuccess) return h->hip_fail(e, "path aggregation launch");
-    e = hipEventRecord(h->ev[slot][3], s);
-    if (e != hipSuccess) return h->hip_fail(e, "event record");
-    e = launch_wta(j, s);
+    e = launch_wta(j, s, nullptr, {ev[5], ev[6]});
     if (e != hipSuccess) return h->hip_fail(e, "WTA launch");
-    e = hipEventRecord(h->ev[slot][4], s);
-    if (e != hipSuccess) return h->hip_fail(e, "event record");
     // The dump hook reads the output from the handle's own buffer: when the caller's buffer is
     // another one, the median writes both.
-    e = launch_post(j, s, dout != h->out.as<int16_t>() ? h->out.as<int16_t>() : nullptr);
+    e = launch_post(j, s, dout != h->out.as<int16_t>() ? h->out.as<int16_t>() : nullptr,
+                    {ev[7], ev[8]});
     if (e != hipSuccess) return h->hip_fail(e, "post-processing launch");
-    e = hipEventRecord(h->ev[slot][5], s);
-    if (e != hipSuccess) return h->hip_fail(e, "event record");
     return LUMEN_OK;
 }
 
@@ -462,14 +470,15 @@ int lumen_sgbm_compute(lumen_sgbm* h, const uint8_t* left, size_t left_stride,
         parallel_copy_rows(pin + n, W, right, right_stride, W, H);
     }
     const int slot = begin_slot(h);
-    e = hipEventRecord(h->ev[slot][0], s);
-    if (e != hipSuccess) return h->hip_fail(e, "event record");
+    if (h->timing >= LUMEN_TIMING_TOTAL) {
+        e = hipEventRecord(h->ev[slot][0], s);
+        if (e != hipSuccess) return h->hip_fail(e, "event record");
+    }
     e = hipMemcpyAsync(h->left.ptr, pin, n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return h->hip_fail(e, "upload left");
     e = hipMemcpyAsync(h->right.ptr, pin + n, n, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return h->hip_fail(e, "upload right");
-    e = hipEventRecord(h->ev[slot][1], s);
-    if (e != hipSuccess) return h->hip_fail(e, "event record");
+    h->slotHasUpload[slot] = true;
     rc = run_stages(h, h->left.as<uint8_t>(), h->right.as<uint8_t>(), h->out.as<int16_t>(), slot);
     if (rc) return rc;
     e = hipMemcpyAsync(h->stageOut.ptr, h->out.ptr, n * 2, hipMemcpyDeviceToHost, s);
@@ -493,10 +502,7 @@ int lumen_sgbm_compute_device(lumen_sgbm* h, const uint8_t* d_left, const uint8_
     h->haveResult = false;
     const int slot = begin_slot(h);
     hipStream_t s = h->stream;
-    for (int k = 0; k < 2; ++k) {
-        hipError_t e = hipEventRecord(h->ev[slot][k], s);
-        if (e != hipSuccess) return h->hip_fail(e, "event record");
-    }
+    h->slotHasUpload[slot] = false;  // the compute starts with the cost kernel (event 1)
     rc = run_stages(h, d_left, d_right, d_out, slot);
     if (rc) return rc;
     if (sync) {
@@ -563,6 +569,14 @@ int lumen_sgbm_compute_device_noalloc(const lumen_stereo_params* params, const u
     return LUMEN_OK;
 }
 
+int lumen_sgbm_set_timing(lumen_sgbm* h, int level) {
+    if (!h) return LUMEN_EINVAL;
+    if (level < LUMEN_TIMING_NONE || level > LUMEN_TIMING_STAGES)
+        return h->fail(LUMEN_EINVAL, "timing level must be 0, 1 or 2");
+    h->timing = level;
+    return LUMEN_OK;
+}
+
 int lumen_sgbm_get_stream(lumen_sgbm* h, void** stream_out) {
     if (!h || !stream_out) return LUMEN_EINVAL;
     int rc = h->ensure_ready();
@@ -572,13 +586,24 @@ int lumen_sgbm_get_stream(lumen_sgbm* h, void** stream_out) {
 }
 
 static int read_slot(lumen_sgbm* h, int slot, float out[6]) {
-    hipError_t e = hipEventSynchronize(h->ev[slot][5]);
+    for (int i = 0; i < 6; ++i) out[i] = 0.0f;
+    const int level = h->slotTiming[slot];
+    if (level == LUMEN_TIMING_NONE) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        return e == hipSuccess ? LUMEN_OK : h->hip_fail(e, "compute");
+    }
+    hipError_t e = hipEventSynchronize(h->ev[slot][8]);
     if (e != hipSuccess) return h->hip_fail(e, "event sync");
-    const int pairs[6][2] = {{0, 5}, {1, 2}, {2, 3}, {4, 5}, {3, 4}, {0, 1}};
+    // Without an upload, the compute starts at event 1 and the upload time is 0.
+    const int s0 = h->slotHasUpload[slot] ? 0 : 1;
+    const int pairs[6][2] = {{s0, 8}, {1, 2}, {3, 4}, {7, 8}, {5, 6}, {s0, 1}};
     for (int i = 0; i < 6; ++i) {
         float ms = 0.0f;
-        e = hipEventElapsedTime(&ms, h->ev[slot][pairs[i][0]], h->ev[slot][pairs[i][1]]);
-        if (e != hipSuccess) return h->hip_fail(e, "event elapsed");
+        const bool recorded = level >= LUMEN_TIMING_STAGES || i == 0 || i == 5;
+        if (recorded && pairs[i][0] != pairs[i][1]) {
+            e = hipEventElapsedTime(&ms, h->ev[slot][pairs[i][0]], h->ev[slot][pairs[i][1]]);
+            if (e != hipSuccess) return h->hip_fail(e, "event elapsed");
+        }
         out[i] = ms;
     }
     return LUMEN_OK;

@@ -1,5 +1,6 @@
 // sgbm_internal.h - shared declarations between the HIP kernels and the host code.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -78,13 +79,23 @@ struct PairJob {
     Schedule sched;
 };
 
-// Per-pair launches (sgbm_kernels.hip).
-hipError_t launch_cost(const PairJob& j, hipStream_t s);
-hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s);
+// Events a stage launch attaches to its own kernel dispatches (hipExtLaunchKernel): `start`
+// takes the start time of the stage's first kernel, `stop` the end time of its last one, so a
+// stage time is the kernels' own duration. Null members are not recorded.
+struct StageEvents {
+    hipEvent_t start = nullptr;
+    hipEvent_t stop = nullptr;
+};
+
+// Per-pair launches (sgbm_kernels.hip, sgbm_paths.hip).
+hipError_t launch_cost(const PairJob& j, hipStream_t s, StageEvents ev = {});
+hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s, StageEvents ev = {});
 // sumOut (may be null): the parity hook's buffer for the fused path sum ([H][W][Dp]).
-hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut = nullptr);
+hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut = nullptr,
+                      StageEvents ev = {});
 // mirror (may be null): a second output buffer the median also writes.
-hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror = nullptr);
+hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror = nullptr,
+                       StageEvents ev = {});
 // [n][Dp] -> [n][D] copy of a padded volume of 1- or 2-byte entries (parity dumps).
 hipError_t launch_strip(const void* src, void* dst, size_t n, int D, int Dp, int elem,
                         hipStream_t s);

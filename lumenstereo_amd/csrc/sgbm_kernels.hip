@@ -15,9 +15,11 @@ __global__ __launch_bounds__(256) void cost_census_kernel(PairJob j) {
 }
 
 template <int T>
-static hipError_t launch_census(const PairJob& j, hipStream_t s) {
+static hipError_t launch_census(const PairJob& j, hipStream_t s, StageEvents ev) {
     dim3 grid((j.W + T - 1) / T, j.H);
-    hipLaunchKernelGGL(cost_census_kernel<T>, grid, dim3(256), census_lds_bytes(j.Dp, T), s, j);
+    hipExtLaunchKernelGGL(cost_census_kernel<T>, grid, dim3(256),
+                          static_cast<uint32_t>(census_lds_bytes(j.Dp, T)), s, ev.start, ev.stop,
+                          0, j);
     return hipGetLastError();
 }
 
@@ -46,28 +48,31 @@ __global__ __launch_bounds__(256) void cost_fill_kernel(PairJob j) {
         dst[i] = u32x4{~0u, ~0u, ~0u, ~0u};
 }
 
-hipError_t launch_cost(const PairJob& j, hipStream_t s) {
+hipError_t launch_cost(const PairJob& j, hipStream_t s, StageEvents ev) {
     if (j.mode == 1) {
         switch (j.sched.censusTile) {
-            case 128: return launch_census<128>(j, s);
-            case 256: return launch_census<256>(j, s);
-            case 512: return launch_census<512>(j, s);
+            case 128: return launch_census<128>(j, s, ev);
+            case 256: return launch_census<256>(j, s, ev);
+            case 512: return launch_census<512>(j, s, ev);
             default: return hipErrorInvalidValue;
         }
     }
     const int r = j.blockSize / 2;
     if (r < 0) {
         // A negative radius gives an empty window: every cost is 255.
-        hipLaunchKernelGGL(cost_fill_kernel, dim3(1024), dim3(256), 0, s, j);
+        hipExtLaunchKernelGGL(cost_fill_kernel, dim3(1024), dim3(256), 0, s, ev.start, ev.stop, 0,
+                              j);
         return hipGetLastError();
     }
     if (sad_lds_bytes(r, j.Dp) <= kSadMaxLds) {
         dim3 grid((j.W + kSadTile - 1) / kSadTile, j.H);
-        hipLaunchKernelGGL(cost_sad_kernel, grid, dim3(256), sad_lds_bytes(r, j.Dp), s, j);
+        hipExtLaunchKernelGGL(cost_sad_kernel, grid, dim3(256),
+                              static_cast<uint32_t>(sad_lds_bytes(r, j.Dp)), s, ev.start, ev.stop,
+                              0, j);
     } else {
         const size_t total = (size_t)j.W * j.H * j.Dp;
-        hipLaunchKernelGGL(cost_sad_direct_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256),
-                           0, s, j);
+        hipExtLaunchKernelGGL(cost_sad_direct_kernel, dim3((unsigned)((total + 255) / 256)),
+                              dim3(256), 0, s, ev.start, ev.stop, 0, j);
     }
     return hipGetLastError();
 }
@@ -93,49 +98,58 @@ __global__ __launch_bounds__(256) void wta_row_dump_kernel(PairJob j, uint16_t* 
 constexpr int kWtaWideRows = 512;
 
 template <int K, int NP>
-static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s, uint16_t* sumOut) {
-    const size_t lds = wta_lds_bytes<K>(j.W);
+static hipError_t launch_wta_kn(const PairJob& j, hipStream_t s, uint16_t* sumOut,
+                                StageEvents ev) {
+    const uint32_t lds = static_cast<uint32_t>(wta_lds_bytes<K>(j.W));
     const bool pad = j.D != j.Dp;
     if (sumOut) {
         if (pad)
-            hipLaunchKernelGGL((wta_row_dump_kernel<K, true, NP>), dim3(j.H), dim3(256), lds, s, j, sumOut);
+            hipExtLaunchKernelGGL((wta_row_dump_kernel<K, true, NP>), dim3(j.H), dim3(256), lds, s,
+                                  ev.start, ev.stop, 0, j, sumOut);
         else
-            hipLaunchKernelGGL((wta_row_dump_kernel<K, false, NP>), dim3(j.H), dim3(256), lds, s, j, sumOut);
+            hipExtLaunchKernelGGL((wta_row_dump_kernel<K, false, NP>), dim3(j.H), dim3(256), lds, s,
+                                  ev.start, ev.stop, 0, j, sumOut);
         return hipGetLastError();
     }
     if constexpr (K <= 8) {
         if (j.H < kWtaWideRows) {
             if (pad)
-                hipLaunchKernelGGL((wta_row_kernel<K, true, NP, 512>), dim3(j.H), dim3(512), lds, s, j);
+                hipExtLaunchKernelGGL((wta_row_kernel<K, true, NP, 512>), dim3(j.H), dim3(512), lds,
+                                      s, ev.start, ev.stop, 0, j);
             else
-                hipLaunchKernelGGL((wta_row_kernel<K, false, NP, 512>), dim3(j.H), dim3(512), lds, s, j);
+                hipExtLaunchKernelGGL((wta_row_kernel<K, false, NP, 512>), dim3(j.H), dim3(512), lds,
+                                      s, ev.start, ev.stop, 0, j);
             return hipGetLastError();
         }
     }
     if (pad) {
-        hipLaunchKernelGGL((wta_row_kernel<K, true, NP, 256>), dim3(j.H), dim3(256), lds, s, j);
+        hipExtLaunchKernelGGL((wta_row_kernel<K, true, NP, 256>), dim3(j.H), dim3(256), lds, s,
+                              ev.start, ev.stop, 0, j);
     } else {
-        hipLaunchKernelGGL((wta_row_kernel<K, false, NP, 256>), dim3(j.H), dim3(256), lds, s, j);
+        hipExtLaunchKernelGGL((wta_row_kernel<K, false, NP, 256>), dim3(j.H), dim3(256), lds, s,
+                              ev.start, ev.stop, 0, j);
     }
     return hipGetLastError();
 }
 
 template <int K>
-static hipError_t launch_wta_k(const PairJob& j, hipStream_t s, uint16_t* sumOut) {
-    return j.npaths == 8 ? launch_wta_kn<K, 8>(j, s, sumOut) : launch_wta_kn<K, 4>(j, s, sumOut);
+static hipError_t launch_wta_k(const PairJob& j, hipStream_t s, uint16_t* sumOut,
+                               StageEvents ev) {
+    return j.npaths == 8 ? launch_wta_kn<K, 8>(j, s, sumOut, ev)
+                         : launch_wta_kn<K, 4>(j, s, sumOut, ev);
 }
 
-hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut) {
+hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut, StageEvents ev) {
     if (j.W > kMaxWidth) return hipErrorInvalidValue;
     switch (j.K) {
-        case 4: return launch_wta_k<4>(j, s, sumOut);
-        case 8: return launch_wta_k<8>(j, s, sumOut);
-        case 12: return launch_wta_k<12>(j, s, sumOut);
-        case 16: return launch_wta_k<16>(j, s, sumOut);
-        case 20: return launch_wta_k<20>(j, s, sumOut);
-        case 24: return launch_wta_k<24>(j, s, sumOut);
-        case 28: return launch_wta_k<28>(j, s, sumOut);
-        case 32: return launch_wta_k<32>(j, s, sumOut);
+        case 4: return launch_wta_k<4>(j, s, sumOut, ev);
+        case 8: return launch_wta_k<8>(j, s, sumOut, ev);
+        case 12: return launch_wta_k<12>(j, s, sumOut, ev);
+        case 16: return launch_wta_k<16>(j, s, sumOut, ev);
+        case 20: return launch_wta_k<20>(j, s, sumOut, ev);
+        case 24: return launch_wta_k<24>(j, s, sumOut, ev);
+        case 28: return launch_wta_k<28>(j, s, sumOut, ev);
+        case 32: return launch_wta_k<32>(j, s, sumOut, ev);
         default: return hipErrorInvalidValue;
     }
 }
@@ -155,15 +169,17 @@ __global__ __launch_bounds__(256) void median3x3_kernel(PairJob j, int16_t* mirr
                 blockIdx.y * kMedTH, tile, mirror ? gmem(mirror) : nullptr);
 }
 
-hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror) {
+hipError_t launch_post(const PairJob& j, hipStream_t s, int16_t* mirror, StageEvents ev) {
+    hipEvent_t first = ev.start;
     if (j.doSpeckle) {
         dim3 grid((j.W + kSpkTW - 1) / kSpkTW, (j.H + kSpkTH - 1) / kSpkTH);
-        hipLaunchKernelGGL(speckle_kernel, grid, dim3(256), 0, s, j);
+        hipExtLaunchKernelGGL(speckle_kernel, grid, dim3(256), 0, s, first, nullptr, 0, j);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
+        first = nullptr;
     }
     dim3 grid((j.W + kMedTW - 1) / kMedTW, (j.H + kMedTH - 1) / kMedTH);
-    hipLaunchKernelGGL(median3x3_kernel, grid, dim3(256), 0, s, j, mirror);
+    hipExtLaunchKernelGGL(median3x3_kernel, grid, dim3(256), 0, s, first, ev.stop, 0, j, mirror);
     return hipGetLastError();
 }
 

@@ -51,27 +51,27 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, 
 
 template <int K, int PF>
 static hipError_t launch_kp(const PairJob& j, const PathsGrid& g, int blocks, uint16_t* trash,
-                            hipStream_t s) {
+                            hipStream_t s, StageEvents ev) {
     if constexpr (!valid_prefetch(K, PF)) {
         return hipErrorInvalidValue;
     } else {
         if (j.D == j.Dp)
-            hipLaunchKernelGGL((sgm_paths_kernel<K, false, PF>), dim3(blocks), dim3(256), 0, s, j,
-                               g, trash);
+            hipExtLaunchKernelGGL((sgm_paths_kernel<K, false, PF>), dim3(blocks), dim3(256), 0, s,
+                                  ev.start, ev.stop, 0, j, g, trash);
         else
-            hipLaunchKernelGGL((sgm_paths_kernel<K, true, PF>), dim3(blocks), dim3(256), 0, s, j,
-                               g, trash);
+            hipExtLaunchKernelGGL((sgm_paths_kernel<K, true, PF>), dim3(blocks), dim3(256), 0, s,
+                                  ev.start, ev.stop, 0, j, g, trash);
         return hipGetLastError();
     }
 }
 
 template <int K>
 static hipError_t launch_k(const PairJob& j, const PathsGrid& g, int blocks, uint16_t* trash,
-                           hipStream_t s) {
+                           hipStream_t s, StageEvents ev) {
     switch (j.sched.aggPrefetch) {
-        case 4: return launch_kp<K, 4>(j, g, blocks, trash, s);
-        case 8: return launch_kp<K, 8>(j, g, blocks, trash, s);
-        case 16: return launch_kp<K, 16>(j, g, blocks, trash, s);
+        case 4: return launch_kp<K, 4>(j, g, blocks, trash, s, ev);
+        case 8: return launch_kp<K, 8>(j, g, blocks, trash, s, ev);
+        case 16: return launch_kp<K, 16>(j, g, blocks, trash, s, ev);
         default: return hipErrorInvalidValue;
     }
 }
@@ -86,7 +86,7 @@ int wide_path_mask(int wide, int W, int H, int npaths) {
     }
 }
 
-hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s) {
+hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s, StageEvents ev) {
     PathsGrid g;
     if (j.sched.aggWide != 0 && !wide_scanlines_ok(j.K, j.sched.aggPrefetch))
         return hipErrorInvalidValue;
@@ -100,14 +100,14 @@ hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s) {
     for (int r = j.npaths; r <= kMaxPaths; ++r) g.waveStart[r] = waves;
     const int blocks = (waves + 3) / 4;
     switch (j.K) {
-        case 4: return launch_k<4>(j, g, blocks, trash, s);
-        case 8: return launch_k<8>(j, g, blocks, trash, s);
-        case 12: return launch_k<12>(j, g, blocks, trash, s);
-        case 16: return launch_k<16>(j, g, blocks, trash, s);
-        case 20: return launch_k<20>(j, g, blocks, trash, s);
-        case 24: return launch_k<24>(j, g, blocks, trash, s);
-        case 28: return launch_k<28>(j, g, blocks, trash, s);
-        case 32: return launch_k<32>(j, g, blocks, trash, s);
+        case 4: return launch_k<4>(j, g, blocks, trash, s, ev);
+        case 8: return launch_k<8>(j, g, blocks, trash, s, ev);
+        case 12: return launch_k<12>(j, g, blocks, trash, s, ev);
+        case 16: return launch_k<16>(j, g, blocks, trash, s, ev);
+        case 20: return launch_k<20>(j, g, blocks, trash, s, ev);
+        case 24: return launch_k<24>(j, g, blocks, trash, s, ev);
+        case 28: return launch_k<28>(j, g, blocks, trash, s, ev);
+        case 32: return launch_k<32>(j, g, blocks, trash, s, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -134,3 +134,26 @@ def test_bgr_input_large(gpu_available):
            rc[..., 2].astype(np.int32) * 4899 + 8192) >> 14).astype(np.uint8)
     m = ls.StereoSGBM(_params(0, 64, 1, 4, {}))
     np.testing.assert_array_equal(m.compute(lc, rc), m.compute(gl, gr))
+
+
+def test_timing_levels(gpu_available):
+    """lumen_sgbm_set_timing: every level computes the same output; level 2 records every stage
+    time, level 1 only the total (getLastComputeTimeMs), level 0 none; an unknown level is
+    refused."""
+    p = _params(0, 64, 1, 8, {})
+    left, right = ls.synth_pair(160, 90, 2, 60, seed=5)
+    m = ls.StereoSGBM(p)
+    want = oracle.sgbm(p, left, right)
+    for level in (2, 1, 0, 2):
+        m.setTiming(level)
+        np.testing.assert_array_equal(m.compute(left, right), want, err_msg=f"level {level}")
+        t = m.timings()
+        if level == 2:
+            assert t[0] > 0 and t[1] > 0 and t[2] > 0 and t[4] > 0 and t[3] > 0
+            assert t[1] + t[2] + t[3] + t[4] <= t[0] * 1.001
+        elif level == 1:
+            assert t[0] > 0 and t[5] > 0 and not (t[1] or t[2] or t[3] or t[4])
+        else:
+            assert not t.any()
+    with pytest.raises(ls.StereoException, match="timing level"):
+        m.setTiming(3)
