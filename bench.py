@@ -217,7 +217,7 @@ def run_plain_fifo(ls, m, trace, entries):
     return lats, (ls.now_ns() - t0) / 1e9
 
 
-def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
+def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes, large_us):
     from lumenstereo_amd import workloads
 
     # Request pool: the 64 mix pairs plus four 2880x1988 D=192 pairs (both cost modes, 4/8 paths).
@@ -241,7 +241,7 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
         m.compute(l, r)
         alone.append((time.perf_counter() - t0) * 1e3)
     free, _ = torch.cuda.mem_get_info()
-    srv = ls.ServeSGBM(pool_bytes=int(free * 0.6), lanes=lanes, max_batch=64)
+    srv = ls.ServeSGBM(pool_bytes=int(free * 0.6), lanes=lanes, max_batch=64, large_us=large_us)
     # Warm-up: every entry once, in one burst and alone, so graphs and staging reach their size.
     for t in [srv.submit(*e) for e in entries]:
         srv.wait(t)
@@ -315,6 +315,7 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes):
                     "probability 0.1 one of four 2880x1988 D=192 pairs; latency from the "
                     "scheduled arrival to the output in host memory",
         "lanes": lanes,
+        "large_us": large_us,
         "sweep": sweep,
         "plain_alone_ms": {"p50": _pct(alone, 50), "p99": _pct(alone, 99),
                            "mix_mean": round(float(np.mean(alone[:len(specs)])), 3),
@@ -560,6 +561,10 @@ def main() -> int:
     ap.add_argument("--serve-rates", default="20,50,100,200,300,400")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-lanes", type=int, default=3)
+    ap.add_argument("--serve-large-us", type=int, default=4500,
+                    help="requests above this much work (microseconds at 5 TB/s of algorithmic "
+                         "traffic) run one at a time on lane 0: the 2880x1988 requests, not the "
+                         "mix's largest pair (1920x1440 D=192 8-path, 4.35 ms)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a plain copy of "
                          "known size, to calibrate rocprof FETCH_SIZE/WRITE_SIZE)")
@@ -638,7 +643,11 @@ def main() -> int:
             if name == args.workload:
                 continue
             print(f"# config {name}", file=sys.stderr, flush=True)
-            c = bench_single(ls, torch, name, 50, 5, 0, 0, parity=not args.no_cpu_baseline,
+            # About a tenth of a second of frames after as many warm-up frames, so a short frame
+            # is not timed while the device is still ramping up.
+            cw, ch = WORKLOADS[name][0], WORKLOADS[name][1]
+            n = max(20, int(0.1 / (cw * ch * WORKLOADS[name][2] * 25 / 4e12)))
+            c = bench_single(ls, torch, name, n, n, 0, 0, parity=not args.no_cpu_baseline,
                              timing=0)
             c.pop("_agg_bytes")
             line["configs"][name] = c
@@ -666,7 +675,8 @@ def main() -> int:
             print("# serving sweep", file=sys.stderr, flush=True)
             rates = [float(x) for x in args.serve_rates.split(",") if x]
             line["serving"] = bench_serving(ls, torch, specs, imgs, plain_outs, rates,
-                                            args.serve_seconds, args.serve_lanes)
+                                            args.serve_seconds, args.serve_lanes,
+                                            args.serve_large_us)
     print(json.dumps(line), flush=True)
     return 0
 
