@@ -95,9 +95,11 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const PairJob* __restri
         const int y = local / segs;
         census_cost_segment(j, y, (local - y * segs) * kCensusTile, lds);
     } else {
-        const int segs = (j.W + kSadTile - 1) / kSadTile;
-        const int y = local / segs;
-        sad_cost_segment(j, y, (local - y * segs) * kSadTile, reinterpret_cast<uint8_t*>(lds));
+        // Consecutive units are consecutive rows of one segment column (the border segments
+        // take longer; this deals them over the XCDs).
+        const int px = sad_segment_px(j.blockSize / 2, j.Dp);
+        const int seg = local / j.H;
+        sad_cost_segment(j, local - seg * j.H, seg * px, reinterpret_cast<uint8_t*>(lds));
     }
 }
 
@@ -315,7 +317,8 @@ void build_batch_plan(BatchPlan& pl) {
             cost = ((j.W + kCensusTile - 1) / kCensusTile) * j.H;
             pl.costLds = std::max(pl.costLds, census_lds_bytes(j.Dp));
         } else if (r >= 0 && sad_lds_bytes(r, j.Dp) <= kSadMaxLds) {
-            cost = ((j.W + kSadTile - 1) / kSadTile) * j.H;
+            const int px = sad_segment_px(r, j.Dp);
+            cost = ((j.W + px - 1) / px) * j.H;
             pl.costLds = std::max(pl.costLds, sad_lds_bytes(r, j.Dp));
         } else {
             pl.special.push_back(p);

@@ -36,6 +36,16 @@ __device__ __forceinline__ u16x2 as16(uint32_t v) { return __builtin_bit_cast(u1
 __device__ __forceinline__ u16x2 splat(uint32_t v) { return as16((v & 0xFFFFu) * 0x10001u); }
 __device__ __forceinline__ u16x2 vmin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
 
+// f(std::integral_constant<int, I>{}) for I = B .. E-1 in order: unrolled steps whose index is
+// a compile-time constant (register ring slots, immediate LDS offsets).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Global-memory view of a pointer. The stage bodies dereference the PairJob's buffers through
 // it: a PairJob read from device memory (the batched kernels' pair table) carries generic
 // pointers, for which the compiler emits flat accesses, and flat accesses also count against
@@ -155,12 +165,32 @@ __host__ __device__ inline size_t census_lds_bytes(int Dp, int T = kCensusTile) 
     return (size_t)kCensusPass * (Dp + 16) + (size_t)(T + nr) * 4 + 5 * (T + 4) + 5 * (nr + 4);
 }
 
+// Windows of radius 1 to 3 (block sizes 3, 5, 7) run a different segment: every thread owns
+// four consecutive disparities and slides over kSadRun pixels, and a 256-thread block covers
+// 1024 / Dp such runs of one row.
+constexpr int kSadRun = 16;
+__host__ __device__ constexpr bool sad_runs_radius(int r) { return r >= 1 && r <= 3; }
+__host__ __device__ inline int sad_segment_px(int r, int Dp) {
+    return sad_runs_radius(r) ? kSadRun * (1024 / Dp) : kSadTile;
+}
+
+// Dword offset of left-tile column c in a run segment: one column of nq dwords after another
+// with a gap of nq dwords every 16 columns, so the (up to four) runs a wave reads at a step,
+// 16 columns apart, fall on different banks.
+__host__ __device__ inline int sad_lcol(int c, int nq) { return nq * (c + (c >> 4)); }
+// Right-tile columns of a run segment sit in four planes by column index mod 4: the threads of
+// a run read columns 4 apart at each step, i.e. consecutive columns of one plane.
+__host__ __device__ inline int sad_plane_dwords(int tr, int nq) { return nq * ((tr + 3) / 4); }
+
 // SAD segment LDS: prefiltered left/right tiles stored column-major (the 2r+1 rows of a column
-// packed four per dword), then the segment's output tile [kSadTile][Dp].
+// packed four per dword); the run segment adds a table of 64 reciprocals, the generic segment
+// its output tile [kSadTile][Dp].
 __host__ __device__ inline size_t sad_lds_bytes(int r, int Dp) {
     const int nq = (2 * r + 1 + 3) / 4;
-    return (size_t)(kSadTile + 2 * r) * nq * 4 + (size_t)(kSadTile + 2 * r + Dp - 1) * nq * 4 +
-           (size_t)kSadTile * Dp;
+    const int P = sad_segment_px(r, Dp), tl = P + 2 * r, tr = tl + Dp - 1;
+    if (sad_runs_radius(r))
+        return (size_t)(sad_lcol(tl - 1, nq) + nq + 4 * sad_plane_dwords(tr, nq) + 64) * 4;
+    return (size_t)tl * nq * 4 + (size_t)tr * nq * 4 + (size_t)P * Dp;
 }
 constexpr size_t kSadMaxLds = 64 * 1024;
 
@@ -188,7 +218,7 @@ __device__ __forceinline__ uint32_t census_at(const uint8_t* rows, int pitch, in
 // at each step are 64 consecutive words (no bank conflicts); the results go to the pass's LDS
 // tile, which the block then stores as one contiguous run of 16-byte pieces.
 template <int T = kCensusTile>
-__device__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* shc) {
+__device__ __forceinline__ void census_cost_segment(const PairJob& j, int y, int x0, uint32_t* shc) {
     const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int rs = x0 - m - (Dp - 1);  // first right pixel the segment can reach
@@ -305,18 +335,280 @@ __device__ __forceinline__ uint8_t prefilter_at(const LUMEN_G uint8_t* __restric
     return static_cast<uint8_t>(g + cap);
 }
 
-// Block-wide; every thread of the block must call it with the same (y, x0); the segment's LDS
-// (sad_lds_bytes) must fit kSadMaxLds.
+// Block-wide; every thread of the block must call it with the same (y, x0): stages the
+// prefiltered tiles of a radius-R segment (R = 1..3). Tile column c < TL is left-image column
+// lbase + c, the others are right-image columns rbase + (c - TL); column c's 2R+1 rows are
+// packed four per dword at dst(c). CPT columns per thread and pass, all their rows' neighbour
+// pairs loaded before the first use (clamped indices, so every load is inside the image): a
+// segment of up to CPT * 256 columns costs one round trip to memory. Clamping the neighbours
+// reproduces prefilter_at's edge rule: column 0 takes row[1] - row[0], column W-1 row[W-1] -
+// row[W-2]. Taps outside the image stage 0 in both tiles.
+template <int R, int CPT, class Dst>
+__device__ __forceinline__ void sad_stage(const PairJob& j, int y, int TL, int TR, int lbase,
+                                          int rbase, int cap, Dst dst) {
+    constexpr int NQ = (2 * R + 1 + 3) / 4;
+    const int W = j.W, H = j.H;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int cb = tid; cb < TL + TR; cb += CPT * nt) {
+        uint32_t a[CPT][2 * R + 1], b[CPT][2 * R + 1];
+#pragma unroll
+        for (int h = 0; h < CPT; ++h) {
+            const int c = min(cb + h * nt, TL + TR - 1);  // a past-the-end column loads a real one
+            const bool isL = c < TL;
+            const int gc = isL ? lbase + c : rbase + (c - TL);
+            const int cl = min(max(gc - 1, 0), W - 1), cr = min(max(gc + 1, 0), W - 1);
+            const LUMEN_G uint8_t* img = gmem(isL ? j.left : j.right);
+#pragma unroll
+            for (int k = 0; k <= 2 * R; ++k) {
+                const size_t row = (size_t)min(max(y - R + k, 0), H - 1) * W;
+                a[h][k] = img[row + cr];
+                b[h][k] = img[row + cl];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < CPT; ++h) {
+            const int c = cb + h * nt;
+            if (c >= TL + TR) break;
+            const int gc = c < TL ? lbase + c : rbase + (c - TL);
+            const bool colOk = gc >= 0 && gc < W;
+            uint32_t w[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) w[q] = 0;
+#pragma unroll
+            for (int k = 0; k <= 2 * R; ++k) {
+                const int yy = y - R + k;
+                const int g = min(cap, max(-cap, static_cast<int>(a[h][k]) - static_cast<int>(b[h][k])));
+                // The prefiltered value is stored as uint8 (src/sgbm_kernels.cu:58): for caps of
+                // 128 and more g + cap can exceed 255 and wraps mod 256, and must not spill into
+                // the next row's byte.
+                const uint32_t v =
+                    (colOk && yy >= 0 && yy < H) ? (static_cast<uint32_t>(g + cap) & 0xFFu) : 0u;
+                w[k / 4] |= v << (8 * (k % 4));
+            }
+            uint32_t* out = dst(c);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) out[q] = w[q];
+        }
+    }
+}
+
+// Window costs of radius R = 1..3 (block sizes 3, 5, 7; src/sgbm_kernels.cu:172-212 with the
+// Sobel-x prefilter of :38-59 fused in) for one row segment of P = sad_segment_px(R, Dp)
+// pixels, block-wide; every thread of the block must call it with the same (y, x0).
 //
-// For disparity d the window sum is a sliding sum over columns: col(c) is the SAD of the
-// 2r+1 staged rows of left column c against right column c - (m+d), two v_sad_u8 per column
-// for r <= 3, and zero when either column is outside the image (its taps do not count). A
-// thread owns one disparity and one part of the segment and slides along x; R > 0 keeps the
-// last 2R+1 columns in registers, R = 0 (any radius) recomputes the column that leaves.
+// Thread (q, part) owns disparities 4q .. 4q+3 and the kSadRun pixels of its part. For each
+// disparity it keeps the running sum P(c) of the column SADs col(c) (v_sad_u8 over the 2R+1
+// staged rows of left column c and the right column c - (m+d)) in a ring of the last 2R+1
+// values: the window of pixel i, left columns i .. i+2R, is P(i+2R) - P(i-1). A column whose
+// right column lies left of the image, or which lies right of it, adds nothing; the valid taps
+// of a window form a rectangle, so their count is rows * columns. A run whose windows all span
+// the full 2R+1 columns skips those checks and divides by the block's window size n (2R+1
+// columns by the rows inside the image) with a multiply: floor(s / n) is byte 3 of
+// s * ceil(2^24 / n) for s <= 255 n and 3 <= n <= 49 (255 n^2 <= 2^24, ceil(2^24 / n) < 2^24,
+// and 255 n * ceil(2^24 / n) < 2^32). The four costs of a pixel are one dword of the volume.
 template <int R>
-__device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs) {
+__device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, uint32_t* lds) {
+    constexpr int NQ = (2 * R + 1 + 3) / 4, NW = 2 * R + 1;
     const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
-    const int r = R > 0 ? R : j.blockSize / 2;
+    const int cap = j.preFilterCap > 1 ? j.preFilterCap : 1;
+    const int runs = 1024 / Dp, P = kSadRun * runs;
+    const int TL = P + 2 * R, TR = TL + Dp - 1;
+    const int PD = sad_plane_dwords(TR, NQ);
+    uint32_t* const lt = lds;
+    uint32_t* const rt = lds + sad_lcol(TL - 1, NQ) + NQ;
+    // recip[8 c + r] = ceil(2^23 / n) for a window of n = c columns by r rows (0 for c = 0):
+    // floor(s / n) = (s * recip) >> 23 for n <= 49 and s <= 255 n (2^23 >= 255 n^2), and
+    // s * recip < 2^31, so the quotient is byte 3 of (2 s) * recip.
+    uint32_t* const recip = rt + 4 * PD;
+    if (threadIdx.x < 64) {
+        const uint32_t n = (threadIdx.x >> 3) * (threadIdx.x & 7);
+        recip[threadIdx.x] = n ? ((1u << 23) + n - 1) / n : 0u;
+    }
+    // Up to 768 columns (Dp up to 448 with R = 3) in one pass.
+    sad_stage<R, 3>(j, y, TL, TR, x0 - R, x0 - R - m - (Dp - 1), cap, [&](int c) -> uint32_t* {
+        if (c < TL) return lt + sad_lcol(c, NQ);
+        const int k = c - TL;
+        return rt + (k & 3) * PD + NQ * (k >> 2);
+    });
+    __syncthreads();
+    const int quads = Dp / 4, tid = threadIdx.x;
+    const int q = tid % quads, part = tid / quads;
+    const int i0 = part * kSadRun;
+    const int npx = min(P, W - x0);
+    if (part >= runs || i0 >= npx) return;
+    const int n = min(kSadRun, npx - i0);
+    const int dq = 4 * q;
+    LUMEN_G uint32_t* const out = reinterpret_cast<LUMEN_G uint32_t*>(
+        gmem(j.cost) + ((size_t)y * W + x0 + i0) * Dp + dq);
+    const int ostep = Dp / 4;  // dwords from one pixel's costs to the next's
+    if (dq >= D) {
+        for (int i = 0; i < n; ++i) out[i * ostep] = ~0u;
+        return;
+    }
+    const uint32_t padMask = D - dq >= 4 ? 0u : (~0u << (8 * (D - dq)));  // d >= D reads 255
+    // Left column i0 + t is at lb + NQ (t + t/16) (i0 is a multiple of 16). Right column
+    // i0 + t + Dp - 1 - dq - k (the one disparity dq + k pairs with left column i0 + t) is
+    // 4 bq + 3 + t - k, which is plane (3 + t - k) & 3 at rb + NQ ((3 + t - k) >> 2).
+    const uint32_t* const lb = lt + NQ * (i0 + part);
+    const int bq = i0 / 4 + quads - 1 - q;
+    auto lcol = [&](int t, uint32_t(&v)[NQ]) {
+        const uint32_t* p = lb + NQ * (t + (t >> 4));
+        if constexpr (NQ == 2) {
+            const u32x2 w = *reinterpret_cast<const u32x2*>(p);
+            v[0] = w.x;
+            v[1] = w.y;
+        } else {
+            v[0] = p[0];
+        }
+    };
+    auto rcol = [&](int e, uint32_t(&v)[NQ]) {  // e = 3 + t - k >= 0
+        const uint32_t* p = rt + (e & 3) * PD + NQ * (bq + (e >> 2));
+        if constexpr (NQ == 2) {
+            const u32x2 w = *reinterpret_cast<const u32x2*>(p);
+            v[0] = w.x;
+            v[1] = w.y;
+        } else {
+            v[0] = p[0];
+        }
+    };
+    // rr[k]: the right column disparity dq + k pairs with the current left column. Before
+    // column t = 0, rr[0..2] hold those of column -1 (shifted into rr[1..3] at t = 0).
+    uint32_t rr[4][NQ];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rcol(2 - k, rr[k]);
+    const int dylo = max(-R, -y), dyhi = min(R, H - 1 - y);
+    const int nrows = dyhi - dylo + 1;
+    const int xf = x0 + i0;
+    const uint32_t nwin = static_cast<uint32_t>(NW * nrows);
+    const uint32_t magic = ((1u << 24) + nwin - 1) / nwin;
+    const bool whole = n == kSadRun && xf - R >= m + dq + 3 && xf + kSadRun - 1 + R <= W - 1;
+    // Waves with border runs take longer: they issue first within their SIMD.
+    if (!__all(whole)) __builtin_amdgcn_s_setprio(2);
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};
+    auto enter = [&](int t, bool checked) {
+        // Left column t enters: rr shifts by one column, then each disparity's running sum
+        // takes the column's SAD (in a checked run, only when the column is a valid tap).
+#pragma unroll
+        for (int k = 3; k > 0; --k)
+#pragma unroll
+            for (int e = 0; e < NQ; ++e) rr[k][e] = rr[k - 1][e];
+        rcol(3 + t, rr[0]);
+        uint32_t lv[NQ];
+        lcol(t, lv);
+        const int gc = xf - R + t;  // image column of left column t
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (checked) {
+                // The column's SAD times 0 or 1: a select here becomes a branch per disparity.
+                uint32_t cs = 0;
+#pragma unroll
+                for (int e = 0; e < NQ; ++e) cs = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], cs);
+                acc[k] += __umul24(cs, static_cast<uint32_t>(gc >= m + dq + k && gc < W));
+            } else {
+#pragma unroll
+                for (int e = 0; e < NQ; ++e) acc[k] = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], acc[k]);
+            }
+        }
+    };
+    auto emit = [&](int i, const uint32_t(&p)[4]) {  // p: each cost in byte 3
+        const uint32_t lo = __builtin_amdgcn_perm(p[1], p[0], 0x0c0c0703u);
+        const uint32_t hi = __builtin_amdgcn_perm(p[3], p[2], 0x0c0c0703u);
+        out[i * ostep] = __builtin_amdgcn_perm(hi, lo, 0x05040100u) | padMask;
+    };
+    // Costs (in byte 3) of pixel x's windows whose taps were checked column by column: the valid
+    // columns and rows form a rectangle, and a window without valid columns reads 255. No
+    // branches: the table read and the product are made for empty windows too (recip 0).
+    auto checked_costs = [&](int x, const uint32_t(&sum)[4], uint32_t(&p)[4]) {
+        const int hi = min(R, W - 1 - x) + 1, b = m + dq - x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int ncols = max(hi - max(-R, b + k), 0);
+            const uint32_t r = recip[(ncols << 3) | nrows];
+            p[k] = __umul24(sum[k] * 2u, r) | (ncols > 0 ? 0u : 0xFF000000u);
+        }
+    };
+    if (n == kSadRun) {
+        // Fully unrolled: the ring slots are registers, the LDS offsets immediates, and the
+        // steps have no branches, so the scheduler overlaps one step's loads with the last
+        // one's arithmetic. A run whose windows are all whole skips the checks.
+        auto run = [&](auto checked) {
+            constexpr bool CHECKED = decltype(checked)::value;
+            uint32_t ring[4][NW];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ring[k][NW - 1] = 0u;  // P(i0 - 1); the others fill first
+            static_for<0, kSadRun + 2 * R>([&](auto tc) {
+                constexpr int t = decltype(tc)::value;
+                enter(t, CHECKED);
+                if constexpr (t >= 2 * R) {
+                    uint32_t p[4];
+                    if constexpr (CHECKED) {
+                        uint32_t sum[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) sum[k] = acc[k] - ring[k][t % NW];
+                        checked_costs(xf + t - 2 * R, sum, p);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) p[k] = __umul24(acc[k] - ring[k][t % NW], magic);
+                    }
+                    emit(t - 2 * R, p);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) ring[k][t % NW] = acc[k];
+            });
+        };
+        if (whole)
+            run(std::integral_constant<bool, false>{});
+        else
+            run(std::integral_constant<bool, true>{});
+    } else {
+        // The short run at the end of a row whose width is not a multiple of 16, rolled.
+        // hist[k] holds P(c - 2R - 1) .. P(c - 1), rotated every step. A step whose lanes all
+        // enter a valid column and emit a window of whole rows' width skips the per-column
+        // checks and divides by the block's window size (fewer rows at the top and bottom of
+        // the image).
+        uint32_t hist[4][NW];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int e = 0; e < NW; ++e) hist[k][e] = 0u;
+#pragma unroll 1
+        for (int t = 0; t < n + 2 * R; ++t) {
+            const int lo = xf - R + t - (t >= 2 * R ? 2 * R : 0);  // leftmost column needed
+            const bool fast = __all(xf - R + t <= W - 1 && lo >= m + dq + 3);
+            enter(t, !fast);
+            if (t >= 2 * R) {
+                uint32_t sum[4], p[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sum[k] = acc[k] - hist[k][0];
+                if (fast) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) p[k] = __umul24(sum[k], magic);
+                } else {
+                    checked_costs(xf + t - 2 * R, sum, p);
+                }
+                emit(t - 2 * R, p);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                for (int e = 0; e < NW - 1; ++e) hist[k][e] = hist[k][e + 1];
+                hist[k][NW - 1] = acc[k];
+            }
+        }
+    }
+}
+
+// Block-wide; every thread of the block must call it with the same (y, x0); the segment's LDS
+// (sad_lds_bytes) must fit kSadMaxLds. Any radius (the run segment covers 1 to 3): one row
+// segment of kSadTile pixels, the prefiltered tiles built byte by byte.
+//
+// For disparity d the window sum is a sliding sum over columns: col(c) is the SAD of the 2r+1
+// staged rows of left column c against right column c - (m+d), and zero when either column is
+// outside the image (its taps do not count). A thread owns one disparity and one part of the
+// segment and slides along x, recomputing the column that leaves.
+__device__ __forceinline__ void sad_cost_segment_generic(const PairJob& j, int y, int x0, uint8_t* shs) {
+    const int W = j.W, H = j.H, D = j.D, Dp = j.Dp, m = j.m;
+    const int r = j.blockSize / 2;
     const int cap = j.preFilterCap > 1 ? j.preFilterCap : 1;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int rows = 2 * r + 1, nq = (rows + 3) / 4;
@@ -326,72 +618,19 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     uint32_t* rt = lt + TL * nq;                          // TR columns x nq dwords
     uint8_t* ot = reinterpret_cast<uint8_t*>(rt + TR * nq);  // [T][Dp]
     const int lbase = x0 - r, rbase = x0 - r - m - (Dp - 1);
-    if constexpr (R > 0) {
-        // One column of the tiles per thread and iteration: the 2R+1 rows' neighbour pairs are
-        // loaded before the first use (clamped indices, so every load is inside the image),
-        // prefiltered, and packed four rows per dword. Clamping the neighbours reproduces
-        // prefilter_at's edge rule: column 0 takes row[1] - row[0], column W-1 row[W-1] -
-        // row[W-2]. Taps outside the image stage 0 in both tiles and add nothing.
-        constexpr int NQ = (2 * R + 1 + 3) / 4;
-        // Two columns per thread and pass (c and c + nt), both columns' loads in flight before
-        // the first use: a segment of up to 2 * nt columns (Dp up to 384) costs one round trip.
-        for (int cb = tid; cb < TL + TR; cb += 2 * nt) {
-            uint32_t a[2][2 * R + 1], b[2][2 * R + 1];
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int c = min(cb + h * nt, TL + TR - 1);  // a past-the-end column loads a real one
-                const bool isL = c < TL;
-                const int gc = isL ? lbase + c : rbase + (c - TL);
-                const int cl = min(max(gc - 1, 0), W - 1), cr = min(max(gc + 1, 0), W - 1);
-                const LUMEN_G uint8_t* img = gmem(isL ? j.left : j.right);
-#pragma unroll
-                for (int k = 0; k <= 2 * R; ++k) {
-                    const size_t row = (size_t)min(max(y - R + k, 0), H - 1) * W;
-                    a[h][k] = img[row + cr];
-                    b[h][k] = img[row + cl];
-                }
-            }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int c = cb + h * nt;
-                if (c >= TL + TR) break;
-                const bool isL = c < TL;
-                const int gc = isL ? lbase + c : rbase + (c - TL);
-                const bool colOk = gc >= 0 && gc < W;
-                uint32_t w[NQ];
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) w[q] = 0;
-#pragma unroll
-                for (int k = 0; k <= 2 * R; ++k) {
-                    const int yy = y - R + k;
-                    const int g = min(cap, max(-cap, static_cast<int>(a[h][k]) - static_cast<int>(b[h][k])));
-                    // The prefiltered value is stored as uint8 (src/sgbm_kernels.cu:58): for
-                    // caps of 128 and more g + cap can exceed 255 and wraps mod 256, and must
-                    // not spill into the next row's byte.
-                    const uint32_t v =
-                        (colOk && yy >= 0 && yy < H) ? (static_cast<uint32_t>(g + cap) & 0xFFu) : 0u;
-                    w[k / 4] |= v << (8 * (k % 4));
-                }
-                uint32_t* dst = isL ? lt + c * NQ : rt + (c - TL) * NQ;
-#pragma unroll
-                for (int q = 0; q < NQ; ++q) dst[q] = w[q];
-            }
-        }
-    } else {
-        uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
-        uint8_t* rb = reinterpret_cast<uint8_t*>(rt);
-        for (int i = tid; i < TL * nq * 4; i += nt) {
-            const int c = i / (nq * 4), k = i - c * (nq * 4);
-            const int gc = lbase + c, yy = y - r + k;
-            lb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
-                        ? prefilter_at(gmem(j.left) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
-        }
-        for (int i = tid; i < TR * nq * 4; i += nt) {
-            const int c = i / (nq * 4), k = i - c * (nq * 4);
-            const int gc = rbase + c, yy = y - r + k;
-            rb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
-                        ? prefilter_at(gmem(j.right) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
-        }
+    uint8_t* lb = reinterpret_cast<uint8_t*>(lt);
+    uint8_t* rb = reinterpret_cast<uint8_t*>(rt);
+    for (int i = tid; i < TL * nq * 4; i += nt) {
+        const int c = i / (nq * 4), k = i - c * (nq * 4);
+        const int gc = lbase + c, yy = y - r + k;
+        lb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
+                    ? prefilter_at(gmem(j.left) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
+    }
+    for (int i = tid; i < TR * nq * 4; i += nt) {
+        const int c = i / (nq * 4), k = i - c * (nq * 4);
+        const int gc = rbase + c, yy = y - r + k;
+        rb[i] = (k < rows && yy >= 0 && yy < H && gc >= 0 && gc < W)
+                    ? prefilter_at(gmem(j.right) + (size_t)yy * W, gc, W, cap) : uint8_t(0);
     }
     __syncthreads();
     const int dylo = max(-r, -y), dyhi = min(r, H - 1 - y);
@@ -419,12 +658,7 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
             const uint32_t* L = lt + c * nq;
             const uint32_t* Rr = rt + (c + Dp - 1 - d) * nq;
             uint32_t acc = 0;
-            if constexpr (R > 0) {
-#pragma unroll
-                for (int q = 0; q < (2 * R + 1 + 3) / 4; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
-            } else {
-                for (int q = 0; q < nq; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
-            }
+            for (int q = 0; q < nq; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
             return acc;
         };
         auto emit = [&](int i, uint32_t sum) {
@@ -441,61 +675,12 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
             ot[i * Dp + d] = static_cast<uint8_t>(v);
         };
         // Window of output pixel i covers tile columns i .. i + 2r.
-        if constexpr (R > 0) {
-            constexpr int NW = 2 * R + 1;
-            // Outputs i in [ilo, ihi) have every tap inside the image and inside the right
-            // image's reachable columns (x - R >= m + d, x + R <= W - 1, all rows present):
-            // their columns need no check and their divisor is the full window.
-            const int ilo = max(i0, a + R - x0);
-            const int ihi = nrows == rows ? min(i1, W - R - x0) : ilo;
-            uint8_t* od = ot + d;
-            auto col_in = [&](int c) -> uint32_t {
-                const uint32_t* L = lt + c * nq;
-                const uint32_t* Rr = rt + (c + Dp - 1 - d) * nq;
-                uint32_t acc = 0;
-#pragma unroll
-                for (int q = 0; q < (2 * R + 1 + 3) / 4; ++q) acc = __builtin_amdgcn_sad_u8(L[q], Rr[q], acc);
-                return acc;
-            };
-            uint32_t ring[NW];
-            uint32_t sum = 0;
-#pragma unroll
-            for (int k = 0; k < NW - 1; ++k) {
-                ring[k] = col(i0 + k);
-                sum += ring[k];
-            }
-            ring[NW - 1] = 0;
-            for (int i = i0; i < i1; i += NW) {
-                if (i >= ilo && i + NW <= ihi) {
-#pragma unroll
-                    for (int u = 0; u < NW; ++u) {
-                        const int slot = (u + NW - 1) % NW;
-                        ring[slot] = col_in(i + u + 2 * R);
-                        sum += ring[slot];
-                        od[(i + u) * Dp] = static_cast<uint8_t>(__umulhi(sum, magic));
-                        sum -= ring[u];
-                    }
-                } else {
-#pragma unroll
-                    for (int u = 0; u < NW; ++u) {
-                        if (i + u < i1) {
-                            const int slot = (u + NW - 1) % NW;  // column i + u + 2R enters here
-                            ring[slot] = col(i + u + 2 * R);
-                            sum += ring[slot];
-                            emit(i + u, sum);
-                            sum -= ring[u];                       // column i + u leaves
-                        }
-                    }
-                }
-            }
-        } else {
-            uint32_t sum = 0;
-            for (int k = 0; k < 2 * r; ++k) sum += col(i0 + k);
-            for (int i = i0; i < i1; ++i) {
-                sum += col(i + 2 * r);
-                emit(i, sum);
-                sum -= col(i);
-            }
+        uint32_t sum = 0;
+        for (int k = 0; k < 2 * r; ++k) sum += col(i0 + k);
+        for (int i = i0; i < i1; ++i) {
+            sum += col(i + 2 * r);
+            emit(i, sum);
+            sum -= col(i);
         }
     }
     __syncthreads();
@@ -507,12 +692,13 @@ __device__ void sad_cost_segment_t(const PairJob& j, int y, int x0, uint8_t* shs
     for (int i = tid; i < n16; i += nt) dst[i] = src[i];
 }
 
-__device__ void sad_cost_segment(const PairJob& j, int y, int x0, uint8_t* shs) {
+// One row segment of sad_segment_px(blockSize / 2, Dp) pixels starting at x0.
+__device__ __forceinline__ void sad_cost_segment(const PairJob& j, int y, int x0, uint8_t* shs) {
     switch (j.blockSize / 2) {
-        case 1: sad_cost_segment_t<1>(j, y, x0, shs); break;
-        case 2: sad_cost_segment_t<2>(j, y, x0, shs); break;
-        case 3: sad_cost_segment_t<3>(j, y, x0, shs); break;
-        default: sad_cost_segment_t<0>(j, y, x0, shs); break;
+        case 1: sad_cost_runs<1>(j, y, x0, reinterpret_cast<uint32_t*>(shs)); break;
+        case 2: sad_cost_runs<2>(j, y, x0, reinterpret_cast<uint32_t*>(shs)); break;
+        case 3: sad_cost_runs<3>(j, y, x0, reinterpret_cast<uint32_t*>(shs)); break;
+        default: sad_cost_segment_generic(j, y, x0, shs); break;
     }
 }
 

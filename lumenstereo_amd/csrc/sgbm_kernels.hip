@@ -23,9 +23,19 @@ static hipError_t launch_census(const PairJob& j, hipStream_t s, StageEvents ev)
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(256) void cost_sad_kernel(PairJob j) {
+// R: the window radius for block sizes 3, 5 and 7, 0 for any other. At least six waves per
+// SIMD (at most 80 VGPRs; four for R = 3, which needs more than 96 to run without spills)
+// keep a whole 640x480 frame's segments resident at once.
+// Block (row, segment): consecutive blocks are consecutive rows, so the segments at the image
+// borders, which take longer, are dealt round-robin over the XCDs like the others.
+template <int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 3 ? 4 : 6))) void cost_sad_kernel(PairJob j) {
     extern __shared__ uint8_t shs[];
-    sad_cost_segment(j, blockIdx.y, blockIdx.x * kSadTile, shs);
+    const int x0 = blockIdx.y * sad_segment_px(R > 0 ? R : j.blockSize / 2, j.Dp);
+    if constexpr (R > 0)
+        sad_cost_runs<R>(j, blockIdx.x, x0, reinterpret_cast<uint32_t*>(shs));
+    else
+        sad_cost_segment_generic(j, blockIdx.x, x0, shs);
 }
 
 // Windows too large for the LDS segment read the images from global memory, one (pixel, d)
@@ -65,10 +75,15 @@ hipError_t launch_cost(const PairJob& j, hipStream_t s, StageEvents ev) {
         return hipGetLastError();
     }
     if (sad_lds_bytes(r, j.Dp) <= kSadMaxLds) {
-        dim3 grid((j.W + kSadTile - 1) / kSadTile, j.H);
-        hipExtLaunchKernelGGL(cost_sad_kernel, grid, dim3(256),
-                              static_cast<uint32_t>(sad_lds_bytes(r, j.Dp)), s, ev.start, ev.stop,
-                              0, j);
+        const int px = sad_segment_px(r, j.Dp);
+        dim3 grid(j.H, (j.W + px - 1) / px);
+        const uint32_t lds = static_cast<uint32_t>(sad_lds_bytes(r, j.Dp));
+        switch (sad_runs_radius(r) ? r : 0) {
+            case 1: hipExtLaunchKernelGGL(cost_sad_kernel<1>, grid, dim3(256), lds, s, ev.start, ev.stop, 0, j); break;
+            case 2: hipExtLaunchKernelGGL(cost_sad_kernel<2>, grid, dim3(256), lds, s, ev.start, ev.stop, 0, j); break;
+            case 3: hipExtLaunchKernelGGL(cost_sad_kernel<3>, grid, dim3(256), lds, s, ev.start, ev.stop, 0, j); break;
+            default: hipExtLaunchKernelGGL(cost_sad_kernel<0>, grid, dim3(256), lds, s, ev.start, ev.stop, 0, j); break;
+        }
     } else {
         const size_t total = (size_t)j.W * j.H * j.Dp;
         hipExtLaunchKernelGGL(cost_sad_direct_kernel, dim3((unsigned)((total + 255) / 256)),
