@@ -61,6 +61,15 @@ public:
         readTimings();
     }
 
+    // Grayscale images and the output in device memory (for example stereo::DeviceBuffer),
+    // width*height bytes / int16 values each, packed. Returns when the output is written.
+    // No reference counterpart: the reference's compute always takes host images.
+    void computeDevice(const uint8_t* d_left, const uint8_t* d_right, int width, int height,
+                       int16_t* d_disparity) {
+        check(lumen_sgbm_compute_device(h_, d_left, d_right, width, height, d_disparity, 1));
+        readTimings();
+    }
+
 #ifdef LUMEN_WITH_OPENCV
     void compute(const cv::Mat& left, const cv::Mat& right, cv::Mat& disparity) {
         if (left.empty() || right.empty()) throw StereoException("Input images are empty");

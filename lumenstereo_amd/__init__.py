@@ -233,6 +233,10 @@ _SIGNATURES = {
     "lumen_sgbm_compute_device_noalloc": (_c_int, [ctypes.POINTER(CParams), _c_void_p, _c_void_p,
                                                    _c_int, _c_int, _c_void_p, _c_void_p]),
     "lumen_sgbm_get_stream": (_c_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
+    "lumen_device_alloc": (_c_int, [_c_size_t, ctypes.POINTER(_c_void_p)]),
+    "lumen_device_free": (_c_int, [_c_void_p]),
+    "lumen_device_copy": (_c_int, [_c_void_p, _c_void_p, _c_size_t, _c_int]),
+    "lumen_device_zero": (_c_int, [_c_void_p, _c_size_t]),
     "lumen_sgbm_get_timings": (_c_int, [_c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "lumen_sgbm_get_timings_avg": (_c_int, [_c_void_p, _c_int, ctypes.POINTER(ctypes.c_float)]),
     "lumen_sgbm_set_timing": (_c_int, [_c_void_p, _c_int]),

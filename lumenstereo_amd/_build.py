@@ -21,7 +21,7 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_paths.hip", "sgbm_capi.hip", "sgbm_batch.hip",
-               "sgbm_serve.hip", "sgbm_tune.hip", "rectify.hip", "depth_map.hip"]
+               "sgbm_serve.hip", "sgbm_tune.hip", "rectify.hip", "depth_map.hip", "device_mem.hip"]
 CXX_SOURCES = ["synth_scene.cpp", "host_copy.cpp", "config.cpp", "evaluate.cpp"]
 
 HIP_FLAGS = [

@@ -8,12 +8,14 @@
 #include <cstdlib>
 #include <functional>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include <cmath>
 #include <fstream>
 
 #include "stereo/depth_map.h"
+#include "stereo/device_buffer.h"
 #include "stereo/rectification.h"
 #include "stereo/sgbm_gpu.h"
 
@@ -208,6 +210,120 @@ static void DifferentSizesAndDisparities() {
     EXPECT(true);
 }
 
+// --- tests/test_sgbm.cpp:81-160 (DeviceBuffer, the reference's CudaBuffer) ----------------
+static void BufferEmpty() {
+    DeviceBuffer<float> buffer;
+    EXPECT(buffer.empty());
+    EXPECT(buffer.size() == 0);
+    EXPECT(buffer.data() == nullptr);
+    // Counts past the end are refused before any device call.
+    float x = 0.0f;
+    EXPECT(throws<StereoException>([&] { buffer.copyFrom(&x, 1); }));
+    EXPECT(throws<StereoException>([&] { buffer.copyTo(&x, 1); }));
+    DeviceBuffer<float> moved(std::move(buffer));
+    EXPECT(moved.empty());
+}
+static void BufferAllocation() {
+    DeviceBuffer<float> buffer(1000);
+    EXPECT(buffer.size() == 1000);
+    EXPECT(buffer.bytes() == 4000);
+    EXPECT(!buffer.empty());
+    EXPECT(buffer.data() != nullptr);
+    float* before = buffer.data();
+    buffer.allocate(1000);  // same size: kept
+    EXPECT(buffer.data() == before);
+    DeviceBuffer<float> moved(std::move(buffer));
+    EXPECT(moved.data() == before && moved.size() == 1000);
+    EXPECT(buffer.empty() && buffer.data() == nullptr);
+    DeviceBuffer<float> assigned;
+    assigned = std::move(moved);
+    EXPECT(assigned.data() == before && moved.empty());
+}
+static void BufferResize() {
+    DeviceBuffer<int> buffer(100);
+    EXPECT(buffer.size() == 100);
+    buffer.resize(500);
+    EXPECT(buffer.size() == 500);
+    buffer.resize(50);
+    EXPECT(buffer.size() == 50);
+    buffer.resize(0);
+    EXPECT(buffer.empty());
+}
+static void BufferZeroMemory() {
+    DeviceBuffer<int> buffer(100);
+    std::vector<int> ones(100, -1);
+    buffer.copyFrom(ones.data());
+    buffer.zero();
+    std::vector<int> host(100, 7);
+    buffer.copyTo(host.data());
+    bool zero = true;
+    for (int v : host) zero &= v == 0;
+    EXPECT(zero);
+}
+static void BufferCopyRoundtrip() {
+    std::vector<float> original = {1.0f, 2.0f, 3.0f, 4.0f, 5.0f};
+    DeviceBuffer<float> buffer(original.size());
+    buffer.copyFrom(original.data());
+    std::vector<float> result(original.size());
+    buffer.copyTo(result.data());
+    EXPECT(result == original);
+    // A partial copy touches only its count.
+    std::vector<float> part = {9.0f, 8.0f};
+    buffer.copyFrom(part.data(), 2);
+    buffer.copyTo(result.data());
+    EXPECT(result[0] == 9.0f && result[1] == 8.0f && result[2] == 3.0f && result[4] == 5.0f);
+    // Device to device into a larger buffer; a larger source is refused.
+    DeviceBuffer<float> big = make2DBuffer<float>(4, 2);
+    EXPECT(big.size() == 8);
+    big.zero();
+    big.copyFromDevice(buffer);
+    std::vector<float> b(8);
+    big.copyTo(b.data());
+    EXPECT(b[0] == 9.0f && b[4] == 5.0f && b[5] == 0.0f && b[7] == 0.0f);
+    EXPECT(throws<StereoException>([&] { buffer.copyFromDevice(big); }));
+}
+static void BufferLargeAllocation() {
+    const size_t size = 100 * 1024 * 1024 / sizeof(float);
+    DeviceBuffer<float> buffer(size);
+    EXPECT(buffer.size() == size);
+}
+static void BufferTooLarge() {
+    // More than any device holds: GpuException with the out-of-memory status, nothing kept.
+    const size_t huge = size_t(1) << 60;
+    bool ok = false;
+    try {
+        DeviceBuffer<uint8_t> buffer(huge);
+    } catch (const GpuException& e) {
+        ok = e.error_code == LUMEN_ENOMEM;
+    }
+    EXPECT(ok);
+    DeviceBuffer<int> after(16);  // the failure leaves no error behind
+    EXPECT(after.size() == 16);
+}
+
+static void DeviceImagesCompute() {
+    // computeDevice on DeviceBuffers equals compute on the same host images.
+    const int W = 160, H = 120;
+    std::vector<uint8_t> l(W * H), r(W * H);
+    std::srand(11);
+    for (int i = 0; i < W * H; ++i) l[i] = static_cast<uint8_t>(std::rand() & 255);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) r[y * W + x] = l[y * W + (x + 7 < W ? x + 7 : W - 1)];
+    StereoParams p;
+    p.maxDisparity = 32;
+    StereoSGBM sgbm(p);
+    std::vector<int16_t> host(W * H), dev(W * H, 12345);
+    sgbm.compute(l.data(), r.data(), W, H, host.data());
+    auto dl = make2DBuffer<uint8_t>(W, H), dr = make2DBuffer<uint8_t>(W, H);
+    auto dd = make2DBuffer<int16_t>(W, H);
+    dl.copyFrom(l.data());
+    dr.copyFrom(r.data());
+    sgbm.computeDevice(dl.data(), dr.data(), W, H, dd.data());
+    dd.copyTo(dev.data());
+    EXPECT(dev == host);
+    EXPECT(sgbm.getLastComputeTimeMs() > 0.0f);
+}
+
 // --- tests/test_rectification.cpp:21-145 (host) ------------------------------------------
 static void CameraDefaultInvalid() {
     CameraParams p;
@@ -361,8 +477,16 @@ int main(int argc, char** argv) {
         {"ConfigTest.StereoParamsSaveLoad", StereoParamsSaveLoad},
         {"ConfigTest.StereoParamsMatchingCostRoundtrip", StereoParamsMatchingCostRoundtrip},
         {"DepthMapTest.DisparityToDepth", DepthFormula},
+        {"DeviceBufferTest.EmptyBuffer", BufferEmpty},
     };
     if (gpu) {
+        tests.push_back({"DeviceBufferTest.Allocation", BufferAllocation});
+        tests.push_back({"DeviceBufferTest.Resize", BufferResize});
+        tests.push_back({"DeviceBufferTest.ZeroMemory", BufferZeroMemory});
+        tests.push_back({"DeviceBufferTest.CopyRoundtrip", BufferCopyRoundtrip});
+        tests.push_back({"DeviceBufferTest.LargeAllocation", BufferLargeAllocation});
+        tests.push_back({"DeviceBufferTest.TooLarge", BufferTooLarge});
+        tests.push_back({"SGBMTest.DeviceImagesCompute", DeviceImagesCompute});
         tests.push_back({"SGBMTest.ComputeBasic", ComputeBasic});
         tests.push_back({"SGBMTest.ColorInput", ColorInput});
         tests.push_back({"SGBMTest.DifferentSizesAndDisparities", DifferentSizesAndDisparities});

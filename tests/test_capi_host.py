@@ -143,3 +143,20 @@ def test_synth_deterministic_and_shaped():
     assert gt.min() >= 4 and gt.max() <= 60
     low = ls.synth_pair(200, 100, 10, 280, seed=5, lowtex_frac=0.2)[0]
     assert low.std() < 40
+
+
+def test_device_memory_argument_checks():
+    """lumen_device_* (the DeviceBuffer ABI) refuse bad arguments before any device call, and
+    treat empty requests as no-ops, as the reference's CudaBuffer does for size 0."""
+    lib = ls.library()
+    p = ctypes.c_void_p(1234)
+    assert lib.lumen_device_alloc(0, ctypes.byref(p)) == ls.LUMEN_OK and p.value is None
+    assert lib.lumen_device_alloc(16, None) == ls.LUMEN_EINVAL
+    assert lib.lumen_device_free(None) == ls.LUMEN_OK
+    assert lib.lumen_device_copy(None, None, 0, 0) == ls.LUMEN_OK
+    assert lib.lumen_device_copy(None, None, 8, 0) == ls.LUMEN_EINVAL
+    buf = (ctypes.c_uint8 * 8)()
+    assert lib.lumen_device_copy(buf, buf, 8, 7) == ls.LUMEN_EINVAL
+    assert b"direction" in lib.lumen_last_error()
+    assert lib.lumen_device_zero(None, 0) == ls.LUMEN_OK
+    assert lib.lumen_device_zero(None, 4) == ls.LUMEN_EINVAL
