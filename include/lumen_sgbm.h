@@ -174,6 +174,10 @@ typedef struct lumen_schedule {
                               step for twice the waves) instead of 16 (four per wave): 0 none,
                               1 the straight pair along the longer image dimension, 2 all four
                               straight paths, 3 every path. */
+    int32_t agg_occupancy; /* aggregation blocks (of four waves) per CU at most: 0 no cap, or
+                              2, 3 or 4 (the kernel then reserves LDS it does not use). Fewer
+                              concurrent scanlines per CU can move a large frame's bytes
+                              faster. */
 } lumen_schedule;
 
 /* Overrides the tuned schedule for this handle (NULL restores it). Checked at compute time:

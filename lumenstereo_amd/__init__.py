@@ -166,7 +166,8 @@ class CSchedule(ctypes.Structure):
     """lumen_schedule (include/lumen_sgbm.h)."""
 
     _fields_ = [("agg_prefetch", ctypes.c_int32), ("agg_priority", ctypes.c_int32),
-                ("census_tile", ctypes.c_int32), ("agg_wide", ctypes.c_int32)]
+                ("census_tile", ctypes.c_int32), ("agg_wide", ctypes.c_int32),
+                ("agg_occupancy", ctypes.c_int32)]
 
 
 class CServerConfig(ctypes.Structure):
@@ -466,12 +467,14 @@ class StereoSGBM:
 
     def setSchedule(self, schedule: dict | None) -> None:
         """Overrides the tuned schedule: keys agg_prefetch, agg_priority, census_tile and
-        optionally agg_wide (None restores the tuned one). Results never depend on it."""
+        optionally agg_wide and agg_occupancy (None restores the tuned one). Results never
+        depend on it."""
         if schedule is None:
             rc = self._lib.lumen_sgbm_set_schedule(self._h, None)
         else:
             c = CSchedule(int(schedule["agg_prefetch"]), int(schedule["agg_priority"]),
-                          int(schedule["census_tile"]), int(schedule.get("agg_wide", 0)))
+                          int(schedule["census_tile"]), int(schedule.get("agg_wide", 0)),
+                          int(schedule.get("agg_occupancy", 0)))
             rc = self._lib.lumen_sgbm_set_schedule(self._h, ctypes.byref(c))
         if rc != LUMEN_OK:
             self._raise(rc)
@@ -483,7 +486,8 @@ class StereoSGBM:
         if rc != LUMEN_OK:
             raise StereoException("invalid size or parameters")
         return {"agg_prefetch": c.agg_prefetch, "agg_priority": c.agg_priority,
-                "census_tile": c.census_tile, "agg_wide": c.agg_wide}
+                "census_tile": c.census_tile, "agg_wide": c.agg_wide,
+                "agg_occupancy": c.agg_occupancy}
 
     # Timing getters of include/stereo/sgbm_gpu.h:52-55.
     def getLastComputeTimeMs(self) -> float:

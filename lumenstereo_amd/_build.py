@@ -20,7 +20,8 @@ ORACLE_LIB = ORACLE_DIR / "liboracle.so"
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_paths.hip", "sgbm_capi.hip", "sgbm_batch.hip",
+HIP_SOURCES = ["sgbm_kernels.hip", "sgbm_paths.hip", "sgbm_paths_k4.hip", "sgbm_paths_k12.hip",
+               "sgbm_paths_k20.hip", "sgbm_paths_k28.hip", "sgbm_capi.hip", "sgbm_batch.hip",
                "sgbm_serve.hip", "sgbm_tune.hip", "rectify.hip", "depth_map.hip", "device_mem.hip"]
 CXX_SOURCES = ["synth_scene.cpp", "host_copy.cpp", "config.cpp", "evaluate.cpp"]
 
@@ -46,7 +47,7 @@ def build_library() -> Path:
     objdir.mkdir(exist_ok=True)
     objs = [objdir / (Path(src).stem + ".o") for src in HIP_SOURCES]
     # The translation units are independent: compile them in parallel (bounded by the CPUs).
-    jobs = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 8))
+    jobs = max(1, min(len(HIP_SOURCES), os.cpu_count() or 1, 16))
     with concurrent.futures.ThreadPoolExecutor(jobs) as pool:
         list(pool.map(lambda so: _run([HIPCC, *HIP_FLAGS, *inc, "-c", CSRC / so[0], "-o", so[1]]),
                       zip(HIP_SOURCES, objs)))

@@ -402,7 +402,8 @@ int lumen_sgbm_set_schedule(lumen_sgbm* h, const lumen_schedule* s) {
         h->hasSchedule = false;
         return LUMEN_OK;
     }
-    h->schedule = Schedule{s->agg_prefetch, s->agg_priority, s->census_tile, s->agg_wide};
+    h->schedule =
+        Schedule{s->agg_prefetch, s->agg_priority, s->census_tile, s->agg_wide, s->agg_occupancy};
     h->hasSchedule = true;
     return LUMEN_OK;
 }
@@ -419,6 +420,7 @@ int lumen_sgbm_get_schedule(const lumen_sgbm* h, int width, int height, lumen_sc
     out->agg_priority = s.aggPriority;
     out->census_tile = s.censusTile;
     out->agg_wide = s.aggWide;
+    out->agg_occupancy = s.aggOccupancy;
     return LUMEN_OK;
 }
 

@@ -34,6 +34,12 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ uint32_t bits(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ u16x2 as16(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
 __device__ __forceinline__ u16x2 splat(uint32_t v) { return as16((v & 0xFFFFu) * 0x10001u); }
+// The low half of v in both halves, as a shuffle: a packed 16-bit consumer reads it through
+// its operand select (op_sel_hi), so no instruction builds the pair.
+__device__ __forceinline__ u16x2 bcast_lo(uint32_t v) {
+    const u16x2 t = as16(v);
+    return __builtin_shufflevector(t, t, 0, 0);
+}
 __device__ __forceinline__ u16x2 vmin(u16x2 a, u16x2 b) { return __builtin_elementwise_min(a, b); }
 
 // f(std::integral_constant<int, I>{}) for I = B .. E-1 in order: unrolled steps whose index is
@@ -999,17 +1005,22 @@ __device__ __forceinline__ uint32_t p2_lookup(const P2Table& t, uint32_t g) {
 // Scanlines s0 .. s0 + 64/LPS - 1 of `path` of pair j (one per LPS lanes); wave-wide (no block
 // synchronisation). Lanes whose scanline does not exist compute on a copy of pixel 0 and store
 // to `trash`. PF is the prefetch depth and LPS the lanes per scanline (schedule choices; the
-// results do not depend on them).
-template <int K, bool PAD, int PF = default_prefetch(K), int LPS = 16>
+// results do not depend on them). HDIR = +1 or -1 compiles a horizontal path (LR or RL) with
+// its step known: consecutive cost and path-cost vectors are then a constant LPS*K (= Dp)
+// elements apart, so a block's loads and stores address off one pointer with immediate
+// offsets instead of a 64-bit add each. HDIR = 0 serves every path with the step at run time.
+template <int K, bool PAD, int PF = default_prefetch(K), int LPS = 16, int HDIR = 0>
 __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_base) {
     static_assert(LPS == 16 || LPS == 32, "a scanline spans one or two DPP rows");
     static_assert(K % 2 == 0, "a lane holds whole packed pairs of disparities");
     static_assert(valid_prefetch(K, PF, LPS), "unsupported prefetch depth for this slice");
+    static_assert(HDIR >= -1 && HDIR <= 1, "HDIR: 0 run-time step, +1 LR, -1 RL");
     const int lane = threadIdx.x & 63;
     const int g = lane / LPS, l = lane & (LPS - 1);
     const int s = s0 + g;
-    int p0, step, len;
-    scan_geom(path, s, j.W, j.H, p0, step, len);
+    int p0, stepv, len;
+    scan_geom(path, s, j.W, j.H, p0, stepv, len);
+    const int step = HDIR != 0 ? HDIR : stepv;
     int maxLen = max(len, __shfl_xor(len, 16));
     maxLen = max(maxLen, __shfl_xor(maxLen, 32));
     maxLen = __builtin_amdgcn_readfirstlane(maxLen);
@@ -1018,7 +1029,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     minLen = __builtin_amdgcn_readfirstlane(minLen);
     const P2Table p2t = make_p2(j.P1, j.P2, lane);
 
-    const int Dp = j.Dp, D = j.D;
+    const int Dp = HDIR != 0 ? LPS * K : j.Dp, D = j.D;  // j.Dp == LPS * K
     constexpr int Q = slice_q<K, LPS>();
     const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
     LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
@@ -1082,6 +1093,11 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
 #pragma unroll
             for (int u = 0; u < PF; ++u) {
                 lp += cstep;
+                // With a run-time step, each load's address is the previous one plus the
+                // step (one add): left to itself the compiler keeps base + u*step for every
+                // u of the block in registers, 2*PF registers that cost waves per SIMD. The
+                // empty asm only makes the pointer opaque; it emits nothing.
+                if constexpr (HDIR == 0) asm volatile("" : "+v"(lp));
                 dst[u] = load_cost<K, LPS>(lp);
             }
             lt += PF;
@@ -1092,8 +1108,9 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             sp += step * Dp;
-            const u16x2 c3 = splat(M + row_bcast(p2v, u));
-            const u16x2 Mp = splat(M);
+            if constexpr (HDIR == 0) asm volatile("" : "+v"(sp));  // as for lp in fetch
+            const u16x2 c3 = bcast_lo(M + row_bcast(p2v, u));
+            const u16x2 Mp = bcast_lo(M);
             u16x2 Ch[K / 2];
             expand_cost<K, LPS>(blk[u], cs, Ch);
             uint32_t dm1[K / 2], dp1[K / 2];

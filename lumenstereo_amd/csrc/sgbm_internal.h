@@ -52,9 +52,10 @@ struct Schedule {
     int aggWide;      // paths on 32-lane scanlines (two per wave): 0 none, 1 the straight
                       // pair along the longer image dimension, 2 all four straight paths,
                       // 3 every path; the others run 16-lane ones
+    int aggOccupancy; // aggregation blocks per CU at most: 0 no cap, 2, 3 or 4
 };
 
-inline Schedule default_schedule(int K) { return Schedule{default_prefetch(K), 1, 256, 0}; }
+inline Schedule default_schedule(int K) { return Schedule{default_prefetch(K), 1, 256, 0, 0}; }
 
 // Everything the device stages need to know about one stereo pair.
 struct PairJob {
@@ -86,6 +87,18 @@ struct StageEvents {
     hipEvent_t start = nullptr;
     hipEvent_t stop = nullptr;
 };
+
+// Wave ranges of the per-pair aggregation grid: waves waveStart[r] .. waveStart[r+1]-1 run
+// path r's scanlines.
+struct PathsGrid {
+    int waveStart[kMaxPaths + 1];
+    int wideMask;  // bit r: path r runs 32-lane scanlines
+};
+
+// The aggregation launch of slice K (sgbm_paths_impl.h; instantiated in sgbm_paths_k*.hip).
+template <int K>
+hipError_t launch_paths_k(const PairJob& j, const PathsGrid& g, int blocks, size_t lds,
+                          uint16_t* trash, hipStream_t s, StageEvents ev);
 
 // Per-pair launches (sgbm_kernels.hip, sgbm_paths.hip).
 hipError_t launch_cost(const PairJob& j, hipStream_t s, StageEvents ev = {});

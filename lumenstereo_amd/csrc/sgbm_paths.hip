@@ -1,80 +1,14 @@
-// sgbm_paths.hip - per-pair SGM aggregation launch: all paths of a pair in one grid.
-//
-// One instantiation per (slice K, padding, prefetch depth): the prefetch depth is a schedule
-// choice (Schedule::aggPrefetch), so each pair runs the instantiation its schedule names.
+// sgbm_paths.hip - per-pair SGM aggregation launch: all paths of a pair in one grid. The
+// kernels of each slice K live in sgbm_paths_impl.h, instantiated by sgbm_paths_k*.hip.
 #include "sgbm_device.h"
+
+#include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace lumen {
 
 int scanline_count(int path, int W, int H) { return scan_count(path, W, H); }
-
-// Wave w takes the next scanlines of the path whose range of waves contains w (paths in order,
-// so the long horizontal scanlines start first): four of 16 lanes each, or, for the paths in
-// wideMask, two of 32 lanes each (half the disparities per lane, so about a third fewer
-// instructions per step on each scanline's dependency chain, for twice the waves).
-struct PathsGrid {
-    int waveStart[kMaxPaths + 1];
-    int wideMask;  // bit r: path r runs 32-lane scanlines
-};
-
-template <int K, bool PAD, int PF>
-__global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, uint16_t* trash) {
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= g.waveStart[j.npaths]) return;
-    int path = 0;
-#pragma unroll
-    for (int r = 1; r < kMaxPaths; ++r)
-        if (r < j.npaths && wave >= g.waveStart[r]) path = r;
-    const bool wide = (g.wideMask >> path) & 1;
-    const int s0 = (wave - g.waveStart[path]) * (wide ? 2 : 4);
-    // The launch ends when its longest scanlines end. Priority 1: the longer straight path
-    // first, then the other straight path, diagonals last; priority 2: by scanline length.
-    if (j.sched.aggPriority == 1) {
-        const bool longest = j.W >= j.H ? path < 2 : (path >= 2 && path < 4);
-        if (longest)
-            __builtin_amdgcn_s_setprio(3);
-        else if (path < 4)
-            __builtin_amdgcn_s_setprio(2);
-        else
-            __builtin_amdgcn_s_setprio(1);
-    } else if (j.sched.aggPriority == 2) {
-        set_priority_for_length(scan_length(path, s0, j.W, j.H));
-    }
-    if constexpr (wide_scanlines_ok(K, PF)) {
-        if (wide) {
-            paths_wave<K / 2, PAD, PF, 32>(j, path, s0, trash);
-            return;
-        }
-    }
-    paths_wave<K, PAD, PF>(j, path, s0, trash);
-}
-
-template <int K, int PF>
-static hipError_t launch_kp(const PairJob& j, const PathsGrid& g, int blocks, uint16_t* trash,
-                            hipStream_t s, StageEvents ev) {
-    if constexpr (!valid_prefetch(K, PF)) {
-        return hipErrorInvalidValue;
-    } else {
-        if (j.D == j.Dp)
-            hipExtLaunchKernelGGL((sgm_paths_kernel<K, false, PF>), dim3(blocks), dim3(256), 0, s,
-                                  ev.start, ev.stop, 0, j, g, trash);
-        else
-            hipExtLaunchKernelGGL((sgm_paths_kernel<K, true, PF>), dim3(blocks), dim3(256), 0, s,
-                                  ev.start, ev.stop, 0, j, g, trash);
-        return hipGetLastError();
-    }
-}
-
-template <int K>
-static hipError_t launch_k(const PairJob& j, const PathsGrid& g, int blocks, uint16_t* trash,
-                           hipStream_t s, StageEvents ev) {
-    switch (j.sched.aggPrefetch) {
-        case 4: return launch_kp<K, 4>(j, g, blocks, trash, s, ev);
-        case 8: return launch_kp<K, 8>(j, g, blocks, trash, s, ev);
-        case 16: return launch_kp<K, 16>(j, g, blocks, trash, s, ev);
-        default: return hipErrorInvalidValue;
-    }
-}
 
 // Paths that run 32-lane scanlines under aggregation lane setting `wide` (Schedule::aggWide).
 int wide_path_mask(int wide, int W, int H, int npaths) {
@@ -84,6 +18,33 @@ int wide_path_mask(int wide, int W, int H, int npaths) {
         case 3: return (1 << npaths) - 1;   // every path
         default: return 0;
     }
+}
+
+// Dynamic LDS per aggregation block that lets at most n blocks share a CU: the LDS of a CU
+// divided by n (at most 64 KiB, the most a launch may request without an attribute), which must
+// still leave too little for an (n+1)-th block. Read once per device.
+static hipError_t occupancy_lds(int n, size_t& lds) {
+    static std::mutex mu;
+    static std::map<int, int> perCu;  // device -> LDS bytes per CU
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    int cu = 0;
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = perCu.find(dev);
+        if (it == perCu.end()) {
+            e = hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev);
+            if (e != hipSuccess) return e;
+            perCu[dev] = cu;
+        } else {
+            cu = it->second;
+        }
+    }
+    const size_t per = std::min<size_t>(65536, static_cast<size_t>(cu) / n) & ~size_t(255);
+    if (per * (n + 1) <= static_cast<size_t>(cu)) return hipErrorInvalidValue;
+    lds = per;
+    return hipSuccess;
 }
 
 hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s, StageEvents ev) {
@@ -99,15 +60,20 @@ hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s, StageE
     }
     for (int r = j.npaths; r <= kMaxPaths; ++r) g.waveStart[r] = waves;
     const int blocks = (waves + 3) / 4;
+    size_t lds = 0;
+    if (j.sched.aggOccupancy != 0) {
+        const hipError_t e = occupancy_lds(j.sched.aggOccupancy, lds);
+        if (e != hipSuccess) return e;
+    }
     switch (j.K) {
-        case 4: return launch_k<4>(j, g, blocks, trash, s, ev);
-        case 8: return launch_k<8>(j, g, blocks, trash, s, ev);
-        case 12: return launch_k<12>(j, g, blocks, trash, s, ev);
-        case 16: return launch_k<16>(j, g, blocks, trash, s, ev);
-        case 20: return launch_k<20>(j, g, blocks, trash, s, ev);
-        case 24: return launch_k<24>(j, g, blocks, trash, s, ev);
-        case 28: return launch_k<28>(j, g, blocks, trash, s, ev);
-        case 32: return launch_k<32>(j, g, blocks, trash, s, ev);
+        case 4: return launch_paths_k<4>(j, g, blocks, lds, trash, s, ev);
+        case 8: return launch_paths_k<8>(j, g, blocks, lds, trash, s, ev);
+        case 12: return launch_paths_k<12>(j, g, blocks, lds, trash, s, ev);
+        case 16: return launch_paths_k<16>(j, g, blocks, lds, trash, s, ev);
+        case 20: return launch_paths_k<20>(j, g, blocks, lds, trash, s, ev);
+        case 24: return launch_paths_k<24>(j, g, blocks, lds, trash, s, ev);
+        case 28: return launch_paths_k<28>(j, g, blocks, lds, trash, s, ev);
+        case 32: return launch_paths_k<32>(j, g, blocks, lds, trash, s, ev);
         default: return hipErrorInvalidValue;
     }
 }

@@ -2,8 +2,8 @@
 // disparity range, cost mode and path count runs with on the per-pair path.
 //
 // The table is a text file written by the autotuner (lumenstereo_amd/tune.py), one entry per
-// line:  width_max  Dp  mode  paths  agg_prefetch  agg_priority  census_tile  [agg_wide]
-// ('#' comments; a missing agg_wide reads as 0).
+// line:  width_max  Dp  mode  paths  agg_prefetch  agg_priority  census_tile  [agg_wide
+// [agg_occupancy]] ('#' comments; a missing agg_wide or agg_occupancy reads as 0).
 // A pair uses the entry of its (Dp, mode, paths) with the smallest width_max >= its width (the
 // largest one when it is wider than all), and the default schedule when there is none. The
 // file next to the library (tuning.tsv) is read on first use; lumen_tuning_load replaces the
@@ -74,19 +74,25 @@ int parse(const std::string& path, std::vector<Entry>& out, std::string& err) {
         if (!(is >> e.wmax)) continue;  // blank or comment line
         if (!(is >> e.dp >> e.mode >> e.paths >> e.s.aggPrefetch >> e.s.aggPriority >>
               e.s.censusTile)) {
-            err = path + ":" + std::to_string(lineNo) + ": expected 7 or 8 integers";
+            err = path + ":" + std::to_string(lineNo) + ": expected 7 to 9 integers";
             return -1;
         }
-        if (!(is >> e.s.aggWide)) {
-            if (!is.eof()) {
-                err = path + ":" + std::to_string(lineNo) + ": expected 7 or 8 integers";
-                return -1;
+        // Optional trailing columns: agg_wide, then agg_occupancy (0 when absent).
+        int* optional[2] = {&e.s.aggWide, &e.s.aggOccupancy};
+        for (int* field : optional) {
+            *field = 0;
+            if (!(is >> *field)) {
+                if (!is.eof()) {
+                    err = path + ":" + std::to_string(lineNo) + ": expected 7 to 9 integers";
+                    return -1;
+                }
+                *field = 0;
+                break;
             }
-            e.s.aggWide = 0;
         }
         std::string extra;
         if (is.clear(), is >> extra) {
-            err = path + ":" + std::to_string(lineNo) + ": expected 7 or 8 integers";
+            err = path + ":" + std::to_string(lineNo) + ": expected 7 to 9 integers";
             return -1;
         }
         std::string why;
@@ -139,6 +145,10 @@ bool valid_schedule(const Schedule& s, int K, std::string& why) {
     }
     if (s.aggWide < 0 || s.aggWide > 3) {
         why = "aggregation lane setting must be 0, 1, 2 or 3";
+        return false;
+    }
+    if (s.aggOccupancy != 0 && (s.aggOccupancy < 2 || s.aggOccupancy > 4)) {
+        why = "aggregation occupancy must be 0, 2, 3 or 4 blocks per CU";
         return false;
     }
     if (s.aggWide != 0 && !wide_scanlines_ok(K, s.aggPrefetch)) {

@@ -10,12 +10,15 @@ and ordered on the device, never what is computed:
 * ``agg_wide``: which aggregation paths run scanlines of 32 lanes (two per wave, half the
   disparities per lane: fewer instructions on each step's dependency chain, twice the waves)
   instead of 16 (four per wave): none, the straight pair along the longer image dimension, all
-  straight paths, or every path.
+  straight paths, or every path;
+* ``agg_occupancy``: at most 2, 3 or 4 aggregation blocks per CU, or no cap (0). A large frame
+  can move its bytes faster with fewer scanlines in flight per CU.
 
 For each tuning key (width bucket, padded disparity range Dp, cost mode, path count) the tuner
 times the candidates on the GPU. It is coordinate descent: first the lanes per scanline and the
 prefetch depth together (they interact: wider scanlines can need deeper prefetch), then the
-priority, on the aggregation stage time; then the census tile on the cost stage time. The candidates of a step
+priority, then the occupancy cap, on the aggregation stage time; then the census tile on the
+cost stage time. The candidates of a step
 are timed round-robin over several rounds and each is scored by its median round, so a slow
 drift of the device's speed (which is as large as the differences being measured, a few per
 cent) falls on every candidate alike instead of favouring whichever was timed last. It checks
@@ -59,7 +62,7 @@ def wide_settings(K: int):
 
 def default_schedule(K: int) -> dict:
     return {"agg_prefetch": 8 if K <= 8 else 4, "agg_priority": 1, "census_tile": 256,
-            "agg_wide": 0}
+            "agg_wide": 0, "agg_occupancy": 0}
 
 
 def agg_candidates(K: int):
@@ -157,8 +160,12 @@ def tune(keys, reps_small: int = 10, rounds: int = 5, log=print):
                               if prio != wb["agg_priority"]]
         times, outs = timed_rounds(m, d_in, (H, W), reps, cands, 2, rounds)
         mismatches += sum(int(not np.array_equal(o, ref)) for o in outs)
-        b = pick(times, 0)
-        best, t_def_agg, best_t = cands[b], wtimes[0], times[b]
+        pb = cands[pick(times, 0)]
+        ocands = [dict(pb)] + [dict(pb, agg_occupancy=o) for o in (2, 3, 4)]
+        otimes, oouts = timed_rounds(m, d_in, (H, W), reps, ocands, 2, rounds)
+        mismatches += sum(int(not np.array_equal(o, ref)) for o in oouts)
+        b = pick(otimes, 0)
+        best, t_def_agg, best_t = ocands[b], wtimes[0], otimes[b]
         tiles = (128, 256, 512) if census else (base["census_tile"],)
         tcands = [dict(best, census_tile=t) for t in tiles]
         ctimes, couts = timed_rounds(m, d_in, (H, W), reps, tcands, 1, rounds if census else 1)
@@ -170,7 +177,7 @@ def tune(keys, reps_small: int = 10, rounds: int = 5, log=print):
             raise RuntimeError(f"{mismatches} schedules changed the output of key {W} {Dp} "
                                f"{int(census)} {paths}")
         entries.append((W, Dp, int(census), paths, best["agg_prefetch"], best["agg_priority"],
-                        best["census_tile"], best["agg_wide"]))
+                        best["census_tile"], best["agg_wide"], best["agg_occupancy"]))
         report.append((W, H, Dp, D, census, paths, t_def_agg, best_t, t_def_cost, best_cost, best))
         log(f"# {W}x{H} Dp={Dp} {'census' if census else 'sad'} {paths}p: agg {t_def_agg:.3f} -> "
             f"{best_t:.3f} ms, cost {t_def_cost:.3f} -> {best_cost:.3f} ms, {best}")
@@ -183,7 +190,8 @@ def write_table(entries, path: Path, command: str) -> None:
     lines = [
         "# Tuned per-pair schedules (written by lumenstereo_amd/tune.py; read by liblumen_sgbm.so).",
         f"# {datetime.date.today().isoformat()}: {command}",
-        "# width_max Dp mode(1=census) paths agg_prefetch agg_priority census_tile agg_wide",
+        "# width_max Dp mode(1=census) paths agg_prefetch agg_priority census_tile agg_wide "
+        "agg_occupancy",
     ]
     for e in entries:
         lines.append(" ".join(str(v) for v in e))
@@ -203,7 +211,7 @@ def write_report(report, path: Path, command: str) -> None:
         lines.append(f"| {W}x{H} | {Dp} ({D}) | {'census' if census else 'SAD'} | {paths} | "
                      f"{ta:.3f} | {tb:.3f} | {ca:.3f} | {cb:.3f} | pf {best['agg_prefetch']}, "
                      f"prio {best['agg_priority']}, tile {best['census_tile']}, "
-                     f"wide {best['agg_wide']} |")
+                     f"wide {best['agg_wide']}, occupancy {best['agg_occupancy']} |")
     path.write_text("\n".join(lines) + "\n")
 
 

@@ -37,12 +37,15 @@ def test_every_candidate_schedule_is_bit_identical(gpu_available, case):
     np.testing.assert_array_equal(ref, oracle.sgbm(p, left, right))
     tiles = (128, 256, 512) if census else (256,)
     n = 0
+    # The occupancy cap cycles through its settings over the combinations.
     for (pf, prio), tile, wide in itertools.product(tune.agg_candidates(K), tiles,
                                                     tune.wide_settings(K)):
+        occ = (0, 2, 3, 4)[n % 4]
         m.setSchedule({"agg_prefetch": pf, "agg_priority": prio, "census_tile": tile,
-                       "agg_wide": wide})
-        np.testing.assert_array_equal(m.compute(left, right), ref,
-                                      err_msg=f"pf {pf} prio {prio} tile {tile} wide {wide}")
+                       "agg_wide": wide, "agg_occupancy": occ})
+        np.testing.assert_array_equal(
+            m.compute(left, right), ref,
+            err_msg=f"pf {pf} prio {prio} tile {tile} wide {wide} occupancy {occ}")
         n += 1
     assert n >= 6
 
@@ -89,6 +92,10 @@ def test_unsupported_schedule_is_refused(gpu_available):
         m.compute(left, right)
     m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 4})
     with pytest.raises(ls.StereoException, match="lane setting"):
+        m.compute(left, right)
+    m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 0,
+                   "agg_occupancy": 1})
+    with pytest.raises(ls.StereoException, match="occupancy"):
         m.compute(left, right)
     m.setSchedule(None)
     np.testing.assert_array_equal(m.compute(left, right), oracle.sgbm(p, left, right))

@@ -51,16 +51,19 @@ def test_override_beats_table(tmp_path):
     m = _handle(128, True, 8)
     m.setSchedule({"agg_prefetch": 4, "agg_priority": 0, "census_tile": 512, "agg_wide": 2})
     assert m.getSchedule(640, 480) == {"agg_prefetch": 4, "agg_priority": 0, "census_tile": 512,
-                                       "agg_wide": 2}
+                                       "agg_wide": 2, "agg_occupancy": 0}
     m.setSchedule(None)
     assert m.getSchedule(640, 480)["agg_prefetch"] == 16
 
 
 @pytest.mark.parametrize("line,msg", [
-    ("640 128 1 8 16 2", "expected 7 or 8 integers"),
-    ("640 128 1 8 16 2 256 1 5", "expected 7 or 8 integers"),
-    ("640 128 1 8 16 2 256 x", "expected 7 or 8 integers"),
+    ("640 128 1 8 16 2", "expected 7 to 9 integers"),
+    ("640 128 1 8 16 2 256 1 2 5", "expected 7 to 9 integers"),
+    ("640 128 1 8 16 2 256 x", "expected 7 to 9 integers"),
+    ("640 128 1 8 16 2 256 1 x", "expected 7 to 9 integers"),
     ("640 128 1 8 16 2 256 4", "lane setting"),
+    ("640 128 1 8 16 2 256 1 1", "occupancy"),
+    ("640 128 1 8 16 2 256 1 5", "occupancy"),
     ("640 100 1 8 8 1 256", "bad width or padded range"),
     ("640 128 2 8 8 1 256", "bad cost mode"),
     ("640 128 1 6 8 1 256", "bad cost mode or path count"),
@@ -118,3 +121,17 @@ def test_lane_setting_column(tmp_path):
             except ls.StereoException:
                 ok = False
             assert ok == tune.valid_prefetch(Dp // 16, pf), (Dp, pf)
+
+
+def test_occupancy_column(tmp_path):
+    """The optional ninth column caps the aggregation's blocks per CU (0 when absent)."""
+    f = tmp_path / "t.tsv"
+    f.write_text("640 128 1 8 16 2 128 3 2\n640 64 1 8 16 2 128 1\n640 192 1 8 8 1 256 2 4\n"
+                 "640 256 1 8 4 1 256 0 3\n")
+    assert ls.tuning_load(str(f)) == 4
+    assert _handle(128, True, 8).getSchedule(640, 480)["agg_occupancy"] == 2
+    assert _handle(64, True, 8).getSchedule(640, 480)["agg_occupancy"] == 0
+    assert _handle(192, True, 8).getSchedule(640, 480)["agg_occupancy"] == 4
+    assert _handle(256, True, 8).getSchedule(640, 480) == {
+        "agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 0,
+        "agg_occupancy": 3}
