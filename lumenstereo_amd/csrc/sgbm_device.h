@@ -1031,6 +1031,8 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
 
     const int Dp = HDIR != 0 ? LPS * K : j.Dp, D = j.D;  // j.Dp == LPS * K
     constexpr int Q = slice_q<K, LPS>();
+    constexpr int kLastChunkWord = (K / Q - 1) * (Q / 2);  // first packed word of the last chunk
+    static_assert(LPS * Q >= 64, "padding must fit in the last chunk");
     const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
     LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
     const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
@@ -1113,21 +1115,29 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
             const u16x2 Mp = bcast_lo(M);
             u16x2 Ch[K / 2];
             expand_cost<K, LPS>(blk[u], cs, Ch);
+            // L(d -+ 1) + P1 are the neighbours of L + P1: one add per word instead of two.
+            u16x2 Lp1[K / 2];
+#pragma unroll
+            for (int q = 0; q < K / 2; ++q) Lp1[q] = Lv[q] + P1p;
             uint32_t dm1[K / 2], dp1[K / 2];
-            neighbours<K, LPS>(Lv, dm1, dp1, l);
+            neighbours<K, LPS>(Lp1, dm1, dp1, l);
 #pragma unroll
             for (int q = 0; q < K / 2; ++q) {
-                uint32_t c1 = bits(as16(dm1[q]) + P1p);
-                uint32_t c2 = bits(as16(dp1[q]) + P1p);
+                uint32_t c1 = dm1[q];
+                uint32_t c2 = dp1[q];
                 if (q == 0) c1 |= m0;
+                // Padding (d >= D) and the last disparity D - 1 lie in the last chunk: a chunk
+                // spans at least 64 disparities and Dp - D < 64. The other words' masks are 0.
                 if constexpr (PAD) {
-                    c2 |= c2m[q];
+                    if (q >= kLastChunkWord) c2 |= c2m[q];
                 } else {
                     if (q == K / 2 - 1) c2 |= c2m[q];
                 }
                 const u16x2 mn = vmin(vmin(Lv[q], as16(c1)), vmin(as16(c2), c3));
                 u16x2 nv = Ch[q] + mn - Mp;
-                if constexpr (PAD) nv = as16(bits(nv) | padm[q]);
+                if constexpr (PAD) {
+                    if (q >= kLastChunkWord) nv = as16(bits(nv) | padm[q]);
+                }
                 Lv[q] = nv;
             }
             if constexpr (decltype(checked)::value)
@@ -1235,8 +1245,10 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
     }
     if constexpr (DUMP) store_path<K>(gmem(sumOut) + (rowPix + x) * j.Dp + l * Q, S);
     if constexpr (PAD) {
+        // Padding lies in the last chunk (at least 64 disparities wide; Dp - D < 64).
+        constexpr int kLastChunkWord = (K / Q - 1) * (Q / 2);
 #pragma unroll
-        for (int q = 0; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
+        for (int q = kLastChunkWord; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
     }
     uint32_t sm1[K / 2], sp1[K / 2];
     neighbours<K>(S, sm1, sp1, l);
