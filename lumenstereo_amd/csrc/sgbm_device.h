@@ -1253,18 +1253,26 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
     uint32_t sm1[K / 2], sp1[K / 2];
     neighbours<K>(S, sm1, sp1, l);
 
-    // Left WTA: (first minimum, second minimum of the multiset).
-    uint32_t k1 = 0xFFFFFFFFu, m2 = 0xFFFFu;
+    // Left WTA: (first minimum, second minimum of the multiset). The first minimum is the
+    // smallest (cost << 16 | d) key; the two smallest costs of the lane's multiset are tracked
+    // on packed pairs (even and odd elements in the two halves, three packed operations per
+    // word), then the halves are merged. keys[] are also the right WTA's upper key words.
+    uint32_t keys[K];
+    uint32_t k1 = 0xFFFFFFFFu;
+    u16x2 a2 = as16(0xFFFFFFFFu), b2 = as16(0xFFFFFFFFu);
 #pragma unroll
     for (int q = 0; q < K / 2; ++q) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
-            const uint32_t key = (v << 16) | static_cast<uint32_t>(slice_d<K>(l, 2 * q + h));
-            m2 = min(m2, max(k1, key) >> 16);
-            k1 = min(k1, key);
+            const uint32_t hiHalf = h ? (bits(S[q]) & 0xFFFF0000u) : (bits(S[q]) << 16);
+            keys[2 * q + h] = hiHalf | static_cast<uint32_t>(slice_d<K>(l, 2 * q + h));
+            k1 = min(k1, keys[2 * q + h]);
         }
+        b2 = vmin(b2, __builtin_elementwise_max(a2, S[q]));
+        a2 = vmin(a2, S[q]);
     }
+    const uint32_t aLo = bits(a2) & 0xFFFFu, aHi = bits(a2) >> 16;
+    uint32_t m2 = min(max(aLo, aHi), min(bits(b2) & 0xFFFFu, bits(b2) >> 16));
 #define LUMEN_WTA_COMBINE(CTRL)                   \
     {                                             \
         const uint32_t ok = dpp<CTRL>(k1);        \
@@ -1297,22 +1305,23 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
         sdl[x] = out;
     }
 
-    // Right WTA candidates.
+    // Right WTA candidates, without branches: every (x, d) is offered. Keys whose cost is
+    // 65535 (padding d >= D, or a real cost of 65535, which the reference's strict minimum
+    // never takes) are ignored when the row is decoded, so they need no test here; an
+    // (x, d) whose right pixel x - m - d is left of the image is offered to pixel 0 with an
+    // upper word of all ones, which no decoded key can lose to.
 #pragma unroll
     for (int q = 0; q < K / 2; ++q) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int d = slice_d<K>(l, 2 * q + h);
-            const int xr = x - m - d;
-            const uint32_t v = h ? (bits(S[q]) >> 16) : (bits(S[q]) & 0xFFFFu);
-            if (d < D && xr >= 0 && v != 0xFFFFu) {
-                const uint32_t lo = h ? (sm1[q] >> 16) : (sm1[q] & 0xFFFFu);
-                const uint32_t hi = h ? (sp1[q] >> 16) : (sp1[q] & 0xFFFFu);
-                const unsigned long long key = (static_cast<unsigned long long>(v) << 48) |
-                                               (static_cast<unsigned long long>(d) << 32) |
-                                               (lo << 16) | hi;
-                atomicMin(&rbest[xr + xr / Q], key);
-            }
+            const int xr = x - m - slice_d<K>(l, 2 * q + h);
+            const uint32_t khi = xr >= 0 ? keys[2 * q + h] : 0xFFFFFFFFu;
+            // S(d-1) << 16 | S(d+1): half h of sm1 and of sp1.
+            const uint32_t klo = h ? __builtin_amdgcn_perm(sm1[q], sp1[q], 0x07060302u)
+                                   : __builtin_amdgcn_perm(sm1[q], sp1[q], 0x05040100u);
+            const int xc = max(xr, 0);
+            atomicMin(&rbest[xc + xc / Q],
+                      (static_cast<unsigned long long>(khi) << 32) | klo);
         }
     }
 }
@@ -1395,7 +1404,7 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest,
     for (int xr = tid; xr < W; xr += nt) {
         const unsigned long long key = rbest[xr + xr / Q];
         float sub = 0.0f;
-        if (key != ~0ull) {
+        if ((key >> 48) != 0xFFFFu) {  // a candidate of cost below 65535
             const int bd = static_cast<int>((key >> 32) & 0xFFFFu);
             sub = static_cast<float>(bd);
             if (bd > 0 && bd < D - 1)
