@@ -83,6 +83,21 @@ def build_cpp_tests() -> Path:
     return CPP_TEST_BIN
 
 
+CVMAT_TEST_SRC = ROOT / "tests" / "cpp" / "test_cvmat_overloads.cpp"
+CVMAT_TEST_BIN = ROOT / "tests" / "cpp" / "test_cvmat_overloads"
+
+
+def build_cvmat_test() -> Path:
+    """The cv::Mat overloads (LUMEN_WITH_OPENCV) against the cv::Mat test double of
+    tests/cpp/mock_opencv: OpenCV itself is not installed."""
+    tmp = CVMAT_TEST_BIN.with_suffix(".tmp")
+    _run(["g++", "-O1", "-std=c++17", "-I", ROOT / "include", "-I",
+          ROOT / "tests" / "cpp" / "mock_opencv", CVMAT_TEST_SRC, "-L", PKG, "-llumen_sgbm",
+          "-Wl,-rpath,$ORIGIN/../../lumenstereo_amd", "-o", tmp])
+    os.replace(tmp, CVMAT_TEST_BIN)
+    return CVMAT_TEST_BIN
+
+
 SERVE_TEST_SRC = ROOT / "tests" / "cpp" / "test_serve_core.cpp"
 SERVE_TEST_BIN = ROOT / "tests" / "cpp" / "test_serve_core"
 
@@ -100,6 +115,7 @@ def build_all() -> None:
     build_oracle()
     build_library()
     build_cpp_tests()
+    build_cvmat_test()
     build_serve_core_test()
 
 
