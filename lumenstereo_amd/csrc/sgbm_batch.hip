@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void batch_cost_kernel(const PairJob* __restri
 }
 
 // ----------------------------------------------------------------------------------- paths
-// KSET 0 handles K in {4, 8, 12}, KSET 1 handles K in {16, ..., 32}: one instantiation's
+// KSET 0 handles K in {4, 6, 8, 12}, KSET 1 handles K in {16, ..., 32}: one instantiation's
 // register count is the largest of its cases, so the big slices get their own kernel.
 template <int KSET>
 __global__ __launch_bounds__(256) void batch_paths_kernel(const PairJob* __restrict__ jobs,
@@ -123,6 +123,7 @@ __global__ __launch_bounds__(256) void batch_paths_kernel(const PairJob* __restr
         if constexpr (KSET == 0) {
             switch (item.K) {
                 case 4: paths_wave<4, true>(j, item.path, item.s0, trash); break;
+                case 6: paths_wave<6, true>(j, item.path, item.s0, trash); break;
                 case 8: paths_wave<8, true>(j, item.path, item.s0, trash); break;
                 default: paths_wave<12, true>(j, item.path, item.s0, trash); break;
             }
@@ -144,6 +145,7 @@ __device__ __forceinline__ void wta_dispatch(const PairJob& j, int y, unsigned l
     if constexpr (KSET == 0) {
         switch (j.K) {
             case 4: wta_row<4, true, NP>(j, y, rbest); break;
+            case 6: wta_row<6, true, NP>(j, y, rbest); break;
             case 8: wta_row<8, true, NP>(j, y, rbest); break;
             default: wta_row<12, true, NP>(j, y, rbest); break;
         }
@@ -262,6 +264,7 @@ hipError_t grow_pinned(void** p, size_t& cap, size_t need) {
 size_t wta_lds_for(int K, int W) {
     switch (K) {
         case 4: return wta_lds_bytes<4>(W);
+        case 6: return wta_lds_bytes<6>(W);
         case 8: return wta_lds_bytes<8>(W);
         case 12: return wta_lds_bytes<12>(W);
         case 16: return wta_lds_bytes<16>(W);

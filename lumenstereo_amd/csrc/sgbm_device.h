@@ -794,11 +794,12 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 // Slice layout. A scanline's disparity vector is spread over LPS lanes (a 16-lane DPP row, or
 // 32 lanes: two rows), lane l holding K of the pixel's Dp = LPS*K disparities.
 //
-// LPS = 16: K/Q chunks of Q consecutive disparities (Q = 8 when K is a multiple of 8, else 4):
-// chunk c of lane l is d = 16Qc + Ql .. 16Qc + Ql + Q-1. A chunk is one 4- or 8-byte cost load
-// and one 8- or 16-byte path-cost access per lane, and the 16 lanes' chunks are contiguous, so
-// every access instruction of a row covers one contiguous 64 to 256 bytes whatever K is. For
-// K <= 8 there is one chunk and lane l simply holds d = lK .. lK+K-1.
+// LPS = 16: K/Q chunks of Q consecutive disparities (Q = 8 when K is a multiple of 8, 4 when
+// it is a multiple of 4, else 2: K = 6, Dp = 96): chunk c of lane l is d = 16Qc + Ql ..
+// 16Qc + Ql + Q-1. A chunk is one 2-, 4- or 8-byte cost load and one 4-, 8- or 16-byte
+// path-cost access per lane, and the 16 lanes' chunks are contiguous, so every access
+// instruction of a row covers one contiguous 32 to 256 bytes whatever K is. For K = 4 and 8
+// there is one chunk and lane l simply holds d = lK .. lK+K-1.
 // LPS = 32: for K = 2, 4, 8 or 16 one chunk, lane l holding d = lK .. lK+K-1; otherwise
 // chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities laid out as above with 32
 // lanes, so each access of a scanline covers 64 to 512 contiguous bytes.
@@ -806,7 +807,7 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 template <int K, int LPS = 16>
 __host__ __device__ constexpr int slice_q() {
     return LPS == 32 ? ((K == 2 || K == 4 || K == 8 || K == 16) ? K : (K % 4 == 0 ? 4 : 2))
-                     : (K % 8 == 0 ? 8 : 4);
+                     : (K % 8 == 0 ? 8 : K % 4 == 0 ? 4 : 2);
 }
 
 template <int K, int LPS = 16>
@@ -1032,7 +1033,8 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     const int Dp = HDIR != 0 ? LPS * K : j.Dp, D = j.D;  // j.Dp == LPS * K
     constexpr int Q = slice_q<K, LPS>();
     constexpr int kLastChunkWord = (K / Q - 1) * (Q / 2);  // first packed word of the last chunk
-    static_assert(LPS * Q >= 64, "padding must fit in the last chunk");
+    // The padding Dp - D is under 64, or under 32 for Dp = 96 (padded_range).
+    static_assert(LPS * Q >= 64 || (LPS == 16 && K == 6), "padding must fit in the last chunk");
     const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
     LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
     const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
@@ -1126,8 +1128,8 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
                 uint32_t c1 = dm1[q];
                 uint32_t c2 = dp1[q];
                 if (q == 0) c1 |= m0;
-                // Padding (d >= D) and the last disparity D - 1 lie in the last chunk: a chunk
-                // spans at least 64 disparities and Dp - D < 64. The other words' masks are 0.
+                // Padding (d >= D) and the last disparity D - 1 lie in the last chunk (see
+                // padded_range: a chunk spans more than Dp - D). The other words' masks are 0.
                 if constexpr (PAD) {
                     if (q >= kLastChunkWord) c2 |= c2m[q];
                 } else {
@@ -1204,13 +1206,21 @@ __device__ __forceinline__ void set_priority_for_length(int len) {
 // minimum (src/sgbm_kernels.cu:748-805), and offers every (pixel, d) to right pixel
 // x - m - d as the 64-bit key cost<<48 | d<<32 | S(d-1)<<16 | S(d+1) through an LDS atomic
 // min. Smallest key = smallest cost, then smallest d, which is the first-minimum rule of the
-// right WTA (src/sgbm_kernels.cu:820-834); costs equal to 65535 are never offered because
-// they never beat the reference's initial value. The LR check (src/sgbm_kernels.cu:862-890)
+// right WTA (src/sgbm_kernels.cu:820-834); keys of cost 65535 never count, as they never beat
+// the reference's initial value. The LR check (src/sgbm_kernels.cu:862-890)
 // only reads the same row, so it runs here too.
 // ---------------------------------------------------------------------------------------
+// LDS slot of right pixel xr in the right-WTA table. The lanes of a DPP row offer keys to
+// right pixels Q apart; with Q = 4 or 8 one padding slot every Q keeps them on distinct banks,
+// with Q = 2 (16-byte stride) they already are.
+template <int Q>
+__host__ __device__ constexpr int wta_slot(int xr) {
+    return Q == 2 ? xr : xr + xr / Q;
+}
+
 template <int K>
 __host__ __device__ inline size_t wta_lds_bytes(int W) {
-    return (size_t)(W + W / slice_q<K>() + 1) * 8 + (size_t)W * 4;
+    return (size_t)(wta_slot<slice_q<K>()>(W) + 1) * 8 + (size_t)W * 4;
 }
 
 template <int K>
@@ -1245,7 +1255,7 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
     }
     if constexpr (DUMP) store_path<K>(gmem(sumOut) + (rowPix + x) * j.Dp + l * Q, S);
     if constexpr (PAD) {
-        // Padding lies in the last chunk (at least 64 disparities wide; Dp - D < 64).
+        // Padding lies in the last chunk, which spans more than Dp - D (padded_range).
         constexpr int kLastChunkWord = (K / Q - 1) * (Q / 2);
 #pragma unroll
         for (int q = kLastChunkWord; q < K / 2; ++q) S[q] = as16(bits(S[q]) | padm[q]);
@@ -1320,7 +1330,7 @@ __device__ __forceinline__ void wta_pixel(const PairJob& j, int x, int l, size_t
             const uint32_t klo = h ? __builtin_amdgcn_perm(sm1[q], sp1[q], 0x07060302u)
                                    : __builtin_amdgcn_perm(sm1[q], sp1[q], 0x05040100u);
             const int xc = max(xr, 0);
-            atomicMin(&rbest[xc + xc / Q],
+            atomicMin(&rbest[wta_slot<Q>(xc)],
                       (static_cast<unsigned long long>(khi) << 32) | klo);
         }
     }
@@ -1344,6 +1354,9 @@ __device__ __forceinline__ void wta_load(const PairJob& j, size_t pix, int l,
                 raw[r][4 * ch + 1] = v.y;
                 raw[r][4 * ch + 2] = v.z;
                 raw[r][4 * ch + 3] = v.w;
+            } else if constexpr (Q == 2) {
+                raw[r][ch] = __builtin_nontemporal_load(
+                    reinterpret_cast<const LUMEN_G uint32_t*>(p + 16 * Q * ch));
             } else {
                 const u32x2 v =
                     __builtin_nontemporal_load(reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch));
@@ -1368,7 +1381,7 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest,
     constexpr int Q = slice_q<K>();
     const int W = j.W, D = j.D, m = j.m;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int nslots = W + W / Q + 1;
+    const int nslots = wta_slot<Q>(W) + 1;
     int16_t* sdl = reinterpret_cast<int16_t*>(rbest + nslots);
     int16_t* sdr = sdl + W;
     for (int i = tid; i < nslots; i += nt) rbest[i] = ~0ull;
@@ -1402,7 +1415,7 @@ __device__ void wta_row(const PairJob& j, int y, unsigned long long* rbest,
     }
     __syncthreads();
     for (int xr = tid; xr < W; xr += nt) {
-        const unsigned long long key = rbest[xr + xr / Q];
+        const unsigned long long key = rbest[wta_slot<Q>(xr)];
         float sub = 0.0f;
         if ((key >> 48) != 0xFFFFu) {  // a candidate of cost below 65535
             const int bd = static_cast<int>((key >> 32) & 0xFFFFu);

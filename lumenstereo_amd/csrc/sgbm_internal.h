@@ -17,15 +17,19 @@ constexpr int kMaxWidth = 4600;
 // Elements of the aggregation kernels' trash buffer (64 lanes x 32 entries).
 constexpr int kTrashElems = 64 * 32;
 
-// Disparity-innermost layouts are padded to a multiple of 64 entries (16 lanes x 4).
-inline int padded_range(int D) { return (D + 63) / 64 * 64; }
+// Disparity-innermost layouts are padded to a multiple of 64 entries (16 lanes x 4), except
+// that ranges of 65 to 96 take 96 (16 lanes x 6, in chunks of two): padding them to 128 would
+// move a third more bytes than the range needs. Every padded range Dp thus leaves less padding
+// (Dp - D) than one chunk of its slice layout spans (64 entries, or 32 for Dp = 96), so the
+// padding always lies in the last chunk.
+inline int padded_range(int D) { return D > 64 && D <= 96 ? 96 : (D + 63) / 64 * 64; }
 
-// Vector stores per aggregation step and lane. 16-lane scanlines: K/8 16-byte stores, or K/4
-// 8-byte ones. 32-lane scanlines: one 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones
+// Vector stores per aggregation step and lane. 16-lane scanlines: K/8 16-byte stores, K/4
+// 8-byte ones, or (K = 6) K/2 4-byte ones. 32-lane scanlines: one 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones
 // (K = 16), K/4 8-byte ones (K = 12) or K/2 4-byte ones (K = 6, 10, 14).
 __host__ __device__ constexpr int path_stores(int K, int LPS = 16) {
     return LPS == 32 ? (K == 2 || K == 4 || K == 8 ? 1 : K == 16 ? 2 : K % 4 == 0 ? K / 4 : K / 2)
-                     : (K % 8 == 0 ? K / 8 : K / 4);
+                     : (K % 8 == 0 ? K / 8 : K % 4 == 0 ? K / 4 : K / 2);
 }
 
 // Prefetch depths the aggregation supports for slice K: each lane of a DPP row fetches the
@@ -36,9 +40,10 @@ __host__ __device__ constexpr bool valid_prefetch(int K, int PF, int LPS = 16) {
 }
 
 // Whether 32-lane scanlines can run slice K (disparities per 16 lanes) with prefetch depth PF:
-// each lane then holds K/2 disparities, and a block's stores must fit the wait count.
+// each lane then holds K/2 disparities, a whole number of packed pairs (not for K = 6), and a
+// block's stores must fit the wait count.
 __host__ __device__ constexpr bool wide_scanlines_ok(int K, int PF) {
-    return K % 2 == 0 && valid_prefetch(K / 2, PF, 32);
+    return K % 4 == 0 && valid_prefetch(K / 2, PF, 32);
 }
 
 __host__ __device__ constexpr int default_prefetch(int K) { return K <= 8 ? 8 : 4; }

@@ -158,6 +158,7 @@ hipError_t launch_wta(const PairJob& j, hipStream_t s, uint16_t* sumOut, StageEv
     if (j.W > kMaxWidth) return hipErrorInvalidValue;
     switch (j.K) {
         case 4: return launch_wta_k<4>(j, s, sumOut, ev);
+        case 6: return launch_wta_k<6>(j, s, sumOut, ev);
         case 8: return launch_wta_k<8>(j, s, sumOut, ev);
         case 12: return launch_wta_k<12>(j, s, sumOut, ev);
         case 16: return launch_wta_k<16>(j, s, sumOut, ev);

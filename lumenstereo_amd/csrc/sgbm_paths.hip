@@ -67,6 +67,7 @@ hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s, StageE
     }
     switch (j.K) {
         case 4: return launch_paths_k<4>(j, g, blocks, lds, trash, s, ev);
+        case 6: return launch_paths_k<6>(j, g, blocks, lds, trash, s, ev);
         case 8: return launch_paths_k<8>(j, g, blocks, lds, trash, s, ev);
         case 12: return launch_paths_k<12>(j, g, blocks, lds, trash, s, ev);
         case 16: return launch_paths_k<16>(j, g, blocks, lds, trash, s, ev);

@@ -45,7 +45,8 @@ std::string library_dir() {
 }
 
 bool valid_entry(const Entry& e, std::string& why) {
-    if (e.wmax < 1 || e.dp < 64 || e.dp % 64 != 0 || e.dp > kMaxDisparityRange) {
+    // A padded range is one padded_range gives: 96, or a multiple of 64.
+    if (e.wmax < 1 || e.dp < 64 || e.dp != padded_range(e.dp) || e.dp > kMaxDisparityRange) {
         why = "bad width or padded range";
         return false;
     }
@@ -121,6 +122,7 @@ bool valid_schedule(const Schedule& s, int K, std::string& why) {
     bool ok = true;
     switch (K) {
         case 4: ok = valid_prefetch(4, s.aggPrefetch); break;
+        case 6: ok = valid_prefetch(6, s.aggPrefetch); break;
         case 8: ok = valid_prefetch(8, s.aggPrefetch); break;
         case 12: ok = valid_prefetch(12, s.aggPrefetch); break;
         case 16: ok = valid_prefetch(16, s.aggPrefetch); break;

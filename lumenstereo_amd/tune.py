@@ -43,8 +43,13 @@ from . import LIB_PATH, MatchingCostMode, StereoParams, StereoSGBM, synth_pair, 
 WIDTHS = {320: 240, 640: 480, 1280: 960, 1920: 1080, 2880: 1988}
 
 
+def padded_range(D: int) -> int:
+    """Mirror of padded_range in sgbm_internal.h: 96 for 65..96, else a multiple of 64."""
+    return 96 if 64 < D <= 96 else (D + 63) // 64 * 64
+
+
 def path_stores(K: int) -> int:
-    return K // 8 if K % 8 == 0 else K // 4
+    return K // 8 if K % 8 == 0 else (K // 4 if K % 4 == 0 else K // 2)
 
 
 def valid_prefetch(K: int, pf: int) -> bool:
@@ -54,10 +59,11 @@ def valid_prefetch(K: int, pf: int) -> bool:
 
 
 def wide_settings(K: int):
-    """agg_wide values a slice of K disparities per 16 lanes supports: every one (32-lane
-    scanlines split any padded range into whole chunks, and store no more per step than
-    16-lane ones, so every valid prefetch depth fits them too)."""
-    return (0, 1, 2, 3)
+    """agg_wide values a slice of K disparities per 16 lanes supports: every one when K is a
+    multiple of 4 (32-lane scanlines split the padded range into whole chunks of packed pairs,
+    and store no more per step than 16-lane ones, so every valid prefetch depth fits them too);
+    only 16-lane scanlines for K = 6 (Dp = 96: three disparities per lane of 32)."""
+    return (0, 1, 2, 3) if K % 4 == 0 else (0,)
 
 
 def default_schedule(K: int) -> dict:
@@ -78,7 +84,7 @@ def tuning_keys(quick: bool = False):
     keys = []
     widths = (640, 1920) if quick else (320, 640, 1280, 1920)
     for W in widths:
-        for Dp in (64, 128, 192, 256):
+        for Dp in (64, 96, 128, 192, 256):
             for census in (False, True):
                 for paths in (4, 8):
                     keys.append((W, WIDTHS[W], Dp, Dp, census, paths))

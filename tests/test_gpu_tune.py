@@ -22,6 +22,8 @@ CASES = [
     (100, 33, 250, False, 8),   # K = 16 padded (32-lane: one chunk of 8)
     (70, 41, 512, True, 4),     # K = 32 (32-lane: one chunk of 16)
     (80, 45, 448, False, 8),    # K = 28 (32-lane: seven chunks of 2)
+    (110, 60, 96, True, 8),     # K = 6 (Dp = 96: three chunks of 2 on 16 lanes; no 32-lane)
+    (90, 70, 75, False, 4),     # K = 6 padded (75 -> 96)
 ]
 
 
@@ -31,7 +33,7 @@ def test_every_candidate_schedule_is_bit_identical(gpu_available, case):
     p = tune.params_for(D, census, paths)
     left, right = ls.synth_pair(W, H, 4, max(5.0, 0.9 * D), seed=W + D)
     m = ls.StereoSGBM(p)
-    K = (D + 63) // 64 * 4
+    K = tune.padded_range(D) // 16
     m.setSchedule(tune.default_schedule(K))
     ref = m.compute(left, right)
     np.testing.assert_array_equal(ref, oracle.sgbm(p, left, right))

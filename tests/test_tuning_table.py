@@ -135,3 +135,25 @@ def test_occupancy_column(tmp_path):
     assert _handle(256, True, 8).getSchedule(640, 480) == {
         "agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 0,
         "agg_occupancy": 3}
+
+
+def test_range_of_96(tmp_path):
+    """Ranges of 65 to 96 pad to 96 (K = 6): a table key of 96 is accepted with 16-lane
+    scanlines and every prefetch depth, and refused with 32-lane ones (three disparities per
+    lane of 32 is not a whole number of packed pairs)."""
+    assert [tune.padded_range(d) for d in (64, 65, 80, 96, 97, 128, 290)] == [64, 96, 96, 96, 128, 128, 320]
+    for pf in (4, 8, 16):
+        f = tmp_path / f"r96_{pf}.tsv"
+        f.write_text(f"640 96 1 8 {pf} 1 256 0\n")
+        assert ls.tuning_load(str(f)) == 1
+        assert tune.valid_prefetch(6, pf)
+    assert _handle(80, True, 8).getSchedule(640, 480)["agg_prefetch"] == 16
+    g = tmp_path / "r96_wide.tsv"
+    g.write_text("640 96 1 8 8 1 256 1\n")
+    with pytest.raises(ls.StereoException, match="32-lane"):
+        ls.tuning_load(str(g))
+    assert tune.wide_settings(6) == (0,)
+    h = tmp_path / "r160.tsv"
+    h.write_text("640 160 1 8 8 1 256\n")
+    with pytest.raises(ls.StereoException, match="padded range"):
+        ls.tuning_load(str(h))
