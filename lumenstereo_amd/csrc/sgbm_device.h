@@ -30,6 +30,10 @@ namespace lumen {
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// Vectors of dwords at 4-byte alignment (plain vector types assume their size, rounded up to
+// a power of two).
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3), aligned(4)));
+typedef unsigned int u32x2a __attribute__((ext_vector_type(2), aligned(4)));
 
 __device__ __forceinline__ uint32_t bits(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ u16x2 as16(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
@@ -795,10 +799,10 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 // 32 lanes: two rows), lane l holding K of the pixel's Dp = LPS*K disparities.
 //
 // LPS = 16: K/Q chunks of Q consecutive disparities (Q = 8 when K is a multiple of 8, 4 when
-// it is a multiple of 4, else 2: K = 6, Dp = 96): chunk c of lane l is d = 16Qc + Ql ..
-// 16Qc + Ql + Q-1. A chunk is one 2-, 4- or 8-byte cost load and one 4-, 8- or 16-byte
-// path-cost access per lane, and the 16 lanes' chunks are contiguous, so every access
-// instruction of a row covers one contiguous 32 to 256 bytes whatever K is. For K = 4 and 8
+// it is a multiple of 4, else K: K = 6, Dp = 96): chunk c of lane l is d = 16Qc + Ql ..
+// 16Qc + Ql + Q-1. A chunk is one 4- or 8-byte cost load (three 2-byte ones for K = 6) and one
+// 8-, 12- or 16-byte path-cost access per lane, and the 16 lanes' chunks are contiguous, so
+// every access instruction of a row covers contiguous bytes whatever K is. For K = 4, 6 and 8
 // there is one chunk and lane l simply holds d = lK .. lK+K-1.
 // LPS = 32: for K = 2, 4, 8 or 16 one chunk, lane l holding d = lK .. lK+K-1; otherwise
 // chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities laid out as above with 32
@@ -807,7 +811,7 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 template <int K, int LPS = 16>
 __host__ __device__ constexpr int slice_q() {
     return LPS == 32 ? ((K == 2 || K == 4 || K == 8 || K == 16) ? K : (K % 4 == 0 ? 4 : 2))
-                     : (K % 8 == 0 ? 8 : K % 4 == 0 ? 4 : 2);
+                     : (K % 8 == 0 ? 8 : K % 4 == 0 ? 4 : K);
 }
 
 template <int K, int LPS = 16>
@@ -828,10 +832,17 @@ __device__ __forceinline__ int slice_elem(int d) {
     return d / (16 * Q) * Q + d % Q;
 }
 
-// A lane's K cost bytes: four to a word, or for chunks of two one chunk per word (low half).
+// Whether a lane's cost bytes are held two to a word (low half): chunks of two, or the
+// six-disparity lane of K = 6; otherwise four to a word.
+template <int K, int LPS = 16>
+__host__ __device__ constexpr bool cost_pairs() {
+    return slice_q<K, LPS>() == 2 || slice_q<K, LPS>() == 6;
+}
+
+// A lane's K cost bytes.
 template <int K, int LPS = 16>
 struct CostBytes {
-    uint32_t w[slice_q<K, LPS>() == 2 ? K / 2 : K / 4];
+    uint32_t w[cost_pairs<K, LPS>() ? K / 2 : K / 4];
 };
 
 // p: the pixel's cost vector + Q*l (the lane's first chunk).
@@ -844,6 +855,17 @@ __device__ __forceinline__ CostBytes<K, LPS> load_cost(const LUMEN_G uint8_t* p)
         const LUMEN_G uint8_t* q = p + LPS * Q * ch;
         if constexpr (Q == 2) {
             c.w[ch] = *reinterpret_cast<const LUMEN_G uint16_t*>(q);
+        } else if constexpr (Q == 6) {
+            // Six bytes at a 2-byte aligned address: the eight bytes from the 4-byte aligned
+            // address at or below it (inside the pixel's 96: lane offsets 6l), split by the
+            // offset. Only the low two bytes of each word are used.
+            const uintptr_t a = reinterpret_cast<uintptr_t>(q);
+            const uint32_t off = static_cast<uint32_t>(a & 3u);  // 0 or 2
+            const u32x2a v = *reinterpret_cast<const LUMEN_G u32x2a*>(q - off);
+            const uint32_t lo = __builtin_amdgcn_alignbyte(v.y, v.x, off);
+            c.w[3 * ch + 0] = lo;
+            c.w[3 * ch + 1] = lo >> 16;
+            c.w[3 * ch + 2] = v.y >> (8 * off);
         } else if constexpr (Q == 4) {
             c.w[ch] = *reinterpret_cast<const LUMEN_G uint32_t*>(q);
         } else if constexpr (Q == 8) {
@@ -873,6 +895,9 @@ __device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)
         const int b = ch * (Q / 2);
         if constexpr (Q == 2) {
             __builtin_nontemporal_store(bits(v[b]), reinterpret_cast<LUMEN_G uint32_t*>(q));
+        } else if constexpr (Q == 6) {
+            __builtin_nontemporal_store(u32x3{bits(v[b]), bits(v[b + 1]), bits(v[b + 2])},
+                                        reinterpret_cast<LUMEN_G u32x3*>(q));
         } else if constexpr (Q == 4) {
             __builtin_nontemporal_store(u32x2{bits(v[b]), bits(v[b + 1])},
                                         reinterpret_cast<LUMEN_G u32x2*>(q));
@@ -890,7 +915,7 @@ __device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)
 template <int K, int LPS = 16>
 __device__ __forceinline__ void expand_cost(const CostBytes<K, LPS>& c, u16x2 cs,
                                             u16x2 (&out)[K / 2]) {
-    if constexpr (slice_q<K, LPS>() == 2) {
+    if constexpr (cost_pairs<K, LPS>()) {
 #pragma unroll
         for (int i = 0; i < K / 2; ++i)
             out[i] = as16(__builtin_amdgcn_perm(0u, c.w[i], 0x0c010c00u)) * cs;
@@ -1034,7 +1059,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     constexpr int Q = slice_q<K, LPS>();
     constexpr int kLastChunkWord = (K / Q - 1) * (Q / 2);  // first packed word of the last chunk
     // The padding Dp - D is under 64, or under 32 for Dp = 96 (padded_range).
-    static_assert(LPS * Q >= 64 || (LPS == 16 && K == 6), "padding must fit in the last chunk");
+    static_assert(LPS * Q >= 64, "padding must fit in the last chunk");
     const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
     LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
     const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
@@ -1212,10 +1237,11 @@ __device__ __forceinline__ void set_priority_for_length(int len) {
 // ---------------------------------------------------------------------------------------
 // LDS slot of right pixel xr in the right-WTA table. The lanes of a DPP row offer keys to
 // right pixels Q apart; with Q = 4 or 8 one padding slot every Q keeps them on distinct banks,
-// with Q = 2 (16-byte stride) they already are.
+// with Q = 6 (48-byte stride: 12 dwords, and 12 l mod 64 differs for the 16 lanes) they
+// already are.
 template <int Q>
 __host__ __device__ constexpr int wta_slot(int xr) {
-    return Q == 2 ? xr : xr + xr / Q;
+    return Q == 6 ? xr : xr + xr / Q;
 }
 
 template <int K>
@@ -1354,9 +1380,12 @@ __device__ __forceinline__ void wta_load(const PairJob& j, size_t pix, int l,
                 raw[r][4 * ch + 1] = v.y;
                 raw[r][4 * ch + 2] = v.z;
                 raw[r][4 * ch + 3] = v.w;
-            } else if constexpr (Q == 2) {
-                raw[r][ch] = __builtin_nontemporal_load(
-                    reinterpret_cast<const LUMEN_G uint32_t*>(p + 16 * Q * ch));
+            } else if constexpr (Q == 6) {
+                const u32x3 v =
+                    __builtin_nontemporal_load(reinterpret_cast<const LUMEN_G u32x3*>(p + 16 * Q * ch));
+                raw[r][3 * ch + 0] = v.x;
+                raw[r][3 * ch + 1] = v.y;
+                raw[r][3 * ch + 2] = v.z;
             } else {
                 const u32x2 v =
                     __builtin_nontemporal_load(reinterpret_cast<const LUMEN_G u32x2*>(p + 16 * Q * ch));

@@ -18,18 +18,18 @@ constexpr int kMaxWidth = 4600;
 constexpr int kTrashElems = 64 * 32;
 
 // Disparity-innermost layouts are padded to a multiple of 64 entries (16 lanes x 4), except
-// that ranges of 65 to 96 take 96 (16 lanes x 6, in chunks of two): padding them to 128 would
-// move a third more bytes than the range needs. Every padded range Dp thus leaves less padding
-// (Dp - D) than one chunk of its slice layout spans (64 entries, or 32 for Dp = 96), so the
-// padding always lies in the last chunk.
+// that ranges of 65 to 96 take 96 (16 lanes x 6): padding them to 128 would move a third more
+// bytes than the range needs. Every padded range Dp thus leaves less padding (Dp - D) than one
+// chunk of its slice layout spans (at least 64 entries), so the padding always lies in the
+// last chunk.
 inline int padded_range(int D) { return D > 64 && D <= 96 ? 96 : (D + 63) / 64 * 64; }
 
 // Vector stores per aggregation step and lane. 16-lane scanlines: K/8 16-byte stores, K/4
-// 8-byte ones, or (K = 6) K/2 4-byte ones. 32-lane scanlines: one 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones
+// 8-byte ones, or (K = 6) one 12-byte one. 32-lane scanlines: one 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones
 // (K = 16), K/4 8-byte ones (K = 12) or K/2 4-byte ones (K = 6, 10, 14).
 __host__ __device__ constexpr int path_stores(int K, int LPS = 16) {
     return LPS == 32 ? (K == 2 || K == 4 || K == 8 ? 1 : K == 16 ? 2 : K % 4 == 0 ? K / 4 : K / 2)
-                     : (K % 8 == 0 ? K / 8 : K % 4 == 0 ? K / 4 : K / 2);
+                     : (K % 8 == 0 ? K / 8 : K % 4 == 0 ? K / 4 : 1);
 }
 
 // Prefetch depths the aggregation supports for slice K: each lane of a DPP row fetches the

@@ -49,7 +49,7 @@ def padded_range(D: int) -> int:
 
 
 def path_stores(K: int) -> int:
-    return K // 8 if K % 8 == 0 else (K // 4 if K % 4 == 0 else K // 2)
+    return K // 8 if K % 8 == 0 else (K // 4 if K % 4 == 0 else 1)
 
 
 def valid_prefetch(K: int, pf: int) -> bool:
