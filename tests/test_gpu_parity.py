@@ -19,7 +19,9 @@ CASES = [
     (130, 77, 3, 43, 1, 4, {}),          # minDisparity > 0
     (160, 90, 0, 64, 1, 8, {"uniquenessRatio": 10}),
     (160, 90, 0, 64, 0, 4, {"disp12MaxDiff": -1}),
-    (120, 80, 0, 96, 0, 4, {"blockSize": 3}),
+    (120, 80, 0, 96, 0, 4, {"blockSize": 3}),        # D = 96: one run of six per lane
+    (130, 70, 5, 85, 1, 8, {"uniquenessRatio": 10}),  # D = 80 padded to 96, minDisparity > 0
+    (4600, 5, 0, 90, 1, 8, {}),           # widest image with the 96 range's right-WTA table
     (120, 80, 0, 48, 0, 4, {"blockSize": 7, "preFilterCap": 15}),
     (100, 70, 0, 32, 1, 4, {"P1": 30000, "P2": 60000}),   # forces uint16 wrap
     (100, 70, 0, 32, 0, 8, {"speckleWindowSize": 20, "speckleRange": 2}),
