@@ -51,6 +51,8 @@ extern "C" int lumen_device_copy(void* dst, const void* src, size_t bytes, int d
         default: return (lumen::set_last_error("unknown copy direction"), LUMEN_EINVAL);
     }
     hipError_t e = hipMemcpy(dst, src, bytes, kind);
+    // A device-to-device copy may return before it is done; the call promises completion.
+    if (e == hipSuccess && kind == hipMemcpyDeviceToDevice) e = hipStreamSynchronize(nullptr);
     return e == hipSuccess ? LUMEN_OK : fail(e, "hipMemcpy");
 }
 
