@@ -268,6 +268,9 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes, lar
             v = [(i[b] - i[a]) / 1e6 for i in infos]
             return {"p50_ms": _pct(v, 50), "p99_ms": _pct(v, 99)}
 
+        lat_big = [lat for lat, (_, e) in zip(lats, trace) if e in big]
+        lat_small = [lat for lat, (_, e) in zip(lats, trace) if e not in big]
+
         sweep.append({
             "rate": rate, "requests": N,
             "served": {"p50_ms": _pct(lats, 50), "p99_ms": _pct(lats, 99),
@@ -278,6 +281,11 @@ def bench_serving(ls, torch, specs, imgs, plain_outs, rates, seconds, lanes, lar
             "p99_served_over_plain": round(_pct(lats, 99) / _pct(plat, 99), 3),
             "p99_served_over_alone_p99": round(_pct(lats, 99) / _pct(alone_tr, 99), 3),
             "slowdown_vs_alone": {"p50": _pct(slow, 50), "p99": _pct(slow, 99)},
+            "served_by_class": {
+                "large": {"requests": len(lat_big), "p50_ms": _pct(lat_big, 50),
+                          "p90_ms": _pct(lat_big, 90)},
+                "small": {"requests": len(lat_small), "p50_ms": _pct(lat_small, 50),
+                          "p99_ms": _pct(lat_small, 99)}},
             "served_phases": {"submit_to_admit": phase("t_submit_ns", "t_admit_ns"),
                               "admit_to_start": phase("t_admit_ns", "t_start_ns"),
                               "start_to_done": phase("t_start_ns", "t_done_ns")},

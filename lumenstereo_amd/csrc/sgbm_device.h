@@ -184,6 +184,19 @@ __host__ __device__ inline int sad_segment_px(int r, int Dp) {
     return sad_runs_radius(r) ? kSadRun * (1024 / Dp) : kSadTile;
 }
 
+// (mul, add) of a window of n taps (0 <= n <= 49) such that floor(s / n) is byte 3 of
+// s * mul + add (mul < 2^24, for a 24-bit multiply) for every s <= 255 n, and 255 for n = 0:
+// - n >= 2: mul = 2 ceil(2^23 / n), add = 0. s * ceil(2^23 / n) / 2^23 exceeds s / n by less
+//   than s / 2^23 <= 255 n / 2^23 < 1 / n (255 n^2 < 2^23), which cannot reach the next
+//   integer, and s * mul <= 255 * 2^24 + 510 n < 2^32.
+// - n = 1: mul = add = 2^24 - 1: s (2^24 - 1) + 2^24 - 1 = s 2^24 + (2^24 - 1 - s).
+// - n = 0: mul = 0, add = 255 << 24 (the window sum is 0).
+__host__ __device__ inline u32x2 sad_div_entry(uint32_t n) {
+    if (n == 0) return u32x2{0u, 0xFF000000u};
+    if (n == 1) return u32x2{0xFFFFFFu, 0xFFFFFFu};
+    return u32x2{2u * (((1u << 23) + n - 1) / n), 0u};
+}
+
 // Dword offset of left-tile column c in a run segment: one column of nq dwords after another
 // with a gap of nq dwords every 16 columns, so the (up to four) runs a wave reads at a step,
 // 16 columns apart, fall on different banks.
@@ -193,13 +206,13 @@ __host__ __device__ inline int sad_lcol(int c, int nq) { return nq * (c + (c >> 
 __host__ __device__ inline int sad_plane_dwords(int tr, int nq) { return nq * ((tr + 3) / 4); }
 
 // SAD segment LDS: prefiltered left/right tiles stored column-major (the 2r+1 rows of a column
-// packed four per dword); the run segment adds a table of 64 reciprocals, the generic segment
-// its output tile [kSadTile][Dp].
+// packed four per dword); the run segment adds a table of 8 divisors (eight-byte aligned), the
+// generic segment its output tile [kSadTile][Dp].
 __host__ __device__ inline size_t sad_lds_bytes(int r, int Dp) {
     const int nq = (2 * r + 1 + 3) / 4;
     const int P = sad_segment_px(r, Dp), tl = P + 2 * r, tr = tl + Dp - 1;
     if (sad_runs_radius(r))
-        return (size_t)(sad_lcol(tl - 1, nq) + nq + 4 * sad_plane_dwords(tr, nq) + 64) * 4;
+        return (size_t)(sad_lcol(tl - 1, nq) + nq + 4 * sad_plane_dwords(tr, nq) + 17) * 4;
     return (size_t)tl * nq * 4 + (size_t)tr * nq * 4 + (size_t)P * Dp;
 }
 constexpr size_t kSadMaxLds = 64 * 1024;
@@ -426,14 +439,13 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
     const int PD = sad_plane_dwords(TR, NQ);
     uint32_t* const lt = lds;
     uint32_t* const rt = lds + sad_lcol(TL - 1, NQ) + NQ;
-    // recip[8 c + r] = ceil(2^23 / n) for a window of n = c columns by r rows (0 for c = 0):
-    // floor(s / n) = (s * recip) >> 23 for n <= 49 and s <= 255 n (2^23 >= 255 n^2), and
-    // s * recip < 2^31, so the quotient is byte 3 of (2 s) * recip.
-    uint32_t* const recip = rt + 4 * PD;
-    if (threadIdx.x < 64) {
-        const uint32_t n = (threadIdx.x >> 3) * (threadIdx.x & 7);
-        recip[threadIdx.x] = n ? ((1u << 23) + n - 1) / n : 0u;
-    }
+    const int dylo = max(-R, -y), dyhi = min(R, H - 1 - y);
+    const int nrows = dyhi - dylo + 1;  // rows of this row's windows inside the image
+    // Divisors of the checked windows: div[c] = (mul, add) for c valid columns, n = c * nrows
+    // taps, with floor(s / n) = byte 3 of s * mul + add for every window sum s <= 255 n
+    // (sad_div_entry); a window without valid columns (s = 0) reads 255.
+    u32x2* const div = reinterpret_cast<u32x2*>(rt + 4 * PD + ((sad_lcol(TL - 1, NQ) + NQ) & 1));
+    if (threadIdx.x < 8) div[threadIdx.x] = sad_div_entry(threadIdx.x * nrows);
     // Up to 768 columns (Dp up to 448 with R = 3) in one pass.
     sad_stage<R, 3>(j, y, TL, TR, x0 - R, x0 - R - m - (Dp - 1), cap, [&](int c) -> uint32_t* {
         if (c < TL) return lt + sad_lcol(c, NQ);
@@ -486,14 +498,15 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
     uint32_t rr[4][NQ];
 #pragma unroll
     for (int k = 0; k < 3; ++k) rcol(2 - k, rr[k]);
-    const int dylo = max(-R, -y), dyhi = min(R, H - 1 - y);
-    const int nrows = dyhi - dylo + 1;
     const int xf = x0 + i0;
     const uint32_t nwin = static_cast<uint32_t>(NW * nrows);
     const uint32_t magic = ((1u << 24) + nwin - 1) / nwin;
-    const bool whole = n == kSadRun && xf - R >= m + dq + 3 && xf + kSadRun - 1 + R <= W - 1;
-    // Waves with border runs take longer: they issue first within their SIMD.
-    if (!__all(whole)) __builtin_amdgcn_s_setprio(2);
+    // The checked form is right for every run; a wave takes it as a whole when any of its runs
+    // needs it (a wave split between the two forms would execute both). Such waves take
+    // longer: they issue first within their SIMD.
+    const bool whole =
+        __all(n == kSadRun && xf - R >= m + dq + 3 && xf + kSadRun - 1 + R <= W - 1);
+    if (!whole) __builtin_amdgcn_s_setprio(2);
     uint32_t acc[4] = {0u, 0u, 0u, 0u};
     auto enter = [&](int t, bool checked) {
         // Left column t enters: rr shifts by one column, then each disparity's running sum
@@ -509,11 +522,11 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (checked) {
-                // The column's SAD times 0 or 1: a select here becomes a branch per disparity.
-                uint32_t cs = 0;
+                uint32_t a = acc[k];
 #pragma unroll
-                for (int e = 0; e < NQ; ++e) cs = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], cs);
-                acc[k] += __umul24(cs, static_cast<uint32_t>(gc >= m + dq + k && gc < W));
+                for (int e = 0; e < NQ; ++e) a = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], a);
+                // Both tests evaluated (no short circuit), so the select stays a select.
+                acc[k] = (gc >= m + dq + k) & (gc < W) ? a : acc[k];
             } else {
 #pragma unroll
                 for (int e = 0; e < NQ; ++e) acc[k] = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], acc[k]);
@@ -526,15 +539,14 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
         out[i * ostep] = __builtin_amdgcn_perm(hi, lo, 0x05040100u) | padMask;
     };
     // Costs (in byte 3) of pixel x's windows whose taps were checked column by column: the valid
-    // columns and rows form a rectangle, and a window without valid columns reads 255. No
-    // branches: the table read and the product are made for empty windows too (recip 0).
+    // columns and rows form a rectangle, so the divisor follows from the valid column count. No
+    // branches: empty windows read their table entry too.
     auto checked_costs = [&](int x, const uint32_t(&sum)[4], uint32_t(&p)[4]) {
         const int hi = min(R, W - 1 - x) + 1, b = m + dq - x;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int ncols = max(hi - max(-R, b + k), 0);
-            const uint32_t r = recip[(ncols << 3) | nrows];
-            p[k] = __umul24(sum[k] * 2u, r) | (ncols > 0 ? 0u : 0xFF000000u);
+            const u32x2 dv = div[max(hi - max(-R, b + k), 0)];
+            p[k] = __umul24(sum[k], dv.x) + dv.y;
         }
     };
     if (n == kSadRun) {
