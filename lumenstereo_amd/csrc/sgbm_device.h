@@ -34,6 +34,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // a power of two).
 typedef unsigned int u32x3 __attribute__((ext_vector_type(3), aligned(4)));
 typedef unsigned int u32x2a __attribute__((ext_vector_type(2), aligned(4)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t bits(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ u16x2 as16(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
@@ -358,17 +359,39 @@ __device__ __forceinline__ uint8_t prefilter_at(const LUMEN_G uint8_t* __restric
     return static_cast<uint8_t>(g + cap);
 }
 
+// Four prefiltered bytes (src/sgbm_kernels.cu:38-59) from the bytes b0..b7 of one image row
+// held in lo (b0..b3) and hi (b4..b7), b0 being the column left of the first: byte j is
+// clamp(b[j+2] - b[j], -cap, cap) + cap mod 256, computed as two packed 16-bit lanes (even and
+// odd j). clampv = min(cap, 255) clamps the same way (|difference| <= 255), addv = cap mod 256
+// gives the same low byte.
+__device__ __forceinline__ uint32_t prefilter4(uint32_t lo, uint32_t hi, i16x2 clampv, i16x2 addv) {
+    const uint32_t mn = __builtin_amdgcn_alignbyte(hi, lo, 2);  // b2..b5
+    const i16x2 m02 = __builtin_bit_cast(i16x2, __builtin_amdgcn_perm(0u, mn, 0x0c020c00u));
+    const i16x2 m13 = __builtin_bit_cast(i16x2, __builtin_amdgcn_perm(0u, mn, 0x0c030c01u));
+    const i16x2 s02 = __builtin_bit_cast(i16x2, __builtin_amdgcn_perm(0u, lo, 0x0c020c00u));
+    const i16x2 s13 = __builtin_bit_cast(i16x2, __builtin_amdgcn_perm(0u, lo, 0x0c030c01u));
+    const i16x2 v02 = __builtin_elementwise_min(__builtin_elementwise_max(m02 - s02, -clampv), clampv) + addv;
+    const i16x2 v13 = __builtin_elementwise_min(__builtin_elementwise_max(m13 - s13, -clampv), clampv) + addv;
+    // Low bytes of the four halves, in column order.
+    return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, v13), __builtin_bit_cast(uint32_t, v02),
+                                 0x06020400u);
+}
+
 // Block-wide; every thread of the block must call it with the same (y, x0): stages the
-// prefiltered tiles of a radius-R segment (R = 1..3). Tile column c < TL is left-image column
-// lbase + c, the others are right-image columns rbase + (c - TL); column c's 2R+1 rows are
-// packed four per dword at dst(c). CPT columns per thread and pass, all their rows' neighbour
+// prefiltered tiles of a radius-R segment (R = 1..3). Left-tile column c (c < TL) is image
+// column lbase + c, right-tile column k (k < TR) image column rbase + k; a column's 2R+1 rows
+// are packed four per dword, at lt + sad_lcol(c, NQ) and at rt + (k & 3) * PD + NQ * (k >> 2).
+// Rows outside the image stage 0.
+//
+// sad_stage_columns: any segment. CPT columns per thread and pass, all their rows' neighbour
 // pairs loaded before the first use (clamped indices, so every load is inside the image): a
 // segment of up to CPT * 256 columns costs one round trip to memory. Clamping the neighbours
 // reproduces prefilter_at's edge rule: column 0 takes row[1] - row[0], column W-1 row[W-1] -
 // row[W-2]. Taps outside the image stage 0 in both tiles.
-template <int R, int CPT, class Dst>
-__device__ __forceinline__ void sad_stage(const PairJob& j, int y, int TL, int TR, int lbase,
-                                          int rbase, int cap, Dst dst) {
+template <int R, int CPT>
+__device__ __forceinline__ void sad_stage_columns(const PairJob& j, int y, int TL, int TR,
+                                                  int lbase, int rbase, int cap, uint32_t* lt,
+                                                  uint32_t* rt, int PD) {
     constexpr int NQ = (2 * R + 1 + 3) / 4;
     const int W = j.W, H = j.H;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -408,9 +431,89 @@ __device__ __forceinline__ void sad_stage(const PairJob& j, int y, int TL, int T
                     (colOk && yy >= 0 && yy < H) ? (static_cast<uint32_t>(g + cap) & 0xFFu) : 0u;
                 w[k / 4] |= v << (8 * (k % 4));
             }
-            uint32_t* out = dst(c);
+            const int k = c - TL;
+            uint32_t* out = c < TL ? lt + sad_lcol(c, NQ) : rt + (k & 3) * PD + NQ * (k >> 2);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) out[q] = w[q];
+        }
+    }
+}
+
+// sad_stage_groups: segments whose every source column lies inside the image, twelve bytes
+// before its row's end (sad_groups_inside), in 4-byte aligned images. Each thread takes a
+// group of four consecutive columns of one tile and loads each of its rows as the three aligned
+// dwords around image columns gc0-1 .. gc0+4 (one round trip for segments of up to 1024
+// columns), forms the four prefiltered bytes with byte selects and packed 16-bit arithmetic
+// (prefilter4), transposes rows into columns with byte permutes and stores the columns inside
+// its tile (the last group of a tile may be partial): about a third of the per-column form's
+// instructions and an eighth of its loads.
+__host__ __device__ inline bool sad_groups_inside(int TL, int TR, int lbase, int rbase, int W) {
+    return lbase >= 1 && lbase + 4 * ((TL + 3) / 4 - 1) + 11 <= W && rbase >= 1 &&
+           rbase + 4 * ((TR + 3) / 4 - 1) + 11 <= W;
+}
+
+template <int R>
+__device__ __forceinline__ void sad_stage_groups(const PairJob& j, int y, int TL, int TR,
+                                                 int lbase, int rbase, int cap, uint32_t* lt,
+                                                 uint32_t* rt, int PD) {
+    constexpr int NQ = (2 * R + 1 + 3) / 4, NR = 2 * R + 1;
+    const int W = j.W, H = j.H;
+    const int GL = (TL + 3) / 4, GR = (TR + 3) / 4;
+    const i16x2 clampv = static_cast<short>(min(cap, 255));
+    const i16x2 addv = static_cast<short>(cap & 0xFF);
+    uint32_t rowOff[NR], rowMask[NR];  // rows outside the image: a clamped row, masked to 0
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int yy = y - R + k;
+        rowMask[k] = yy >= 0 && yy < H ? ~0u : 0u;
+        rowOff[k] = static_cast<uint32_t>(min(max(yy, 0), H - 1)) * static_cast<uint32_t>(W);
+    }
+    for (int gi = threadIdx.x; gi < GL + GR; gi += blockDim.x) {
+        const bool isL = gi < GL;
+        const int c0 = 4 * (isL ? gi : gi - GL);  // first column of the group in its tile
+        const int n = isL ? TL : TR;
+        const int gc0 = (isL ? lbase : rbase) + c0;
+        const LUMEN_G uint8_t* img = gmem(isL ? j.left : j.right);
+        // Column c0 + t of the group is at col0 + t * cstep (the four columns of a left group
+        // share one 16-column run, those of a right group one index in the four planes).
+        uint32_t* const col0 = isL ? lt + sad_lcol(c0, NQ) : rt + NQ * (c0 >> 2);
+        const int cstep = isL ? NQ : PD;
+        u32x3 w[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const uint32_t a = rowOff[k] + static_cast<uint32_t>(gc0 - 1);
+            w[k] = *reinterpret_cast<const LUMEN_G u32x3*>(img + (a & ~3u));
+        }
+        uint32_t P[4 * NQ];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const uint32_t off = (rowOff[k] + static_cast<uint32_t>(gc0 - 1)) & 3u;
+            const uint32_t lo = __builtin_amdgcn_alignbyte(w[k].y, w[k].x, off);
+            const uint32_t hi = __builtin_amdgcn_alignbyte(w[k].z, w[k].y, off);
+            P[k] = prefilter4(lo, hi, clampv, addv) & rowMask[k];
+        }
+#pragma unroll
+        for (int k = NR; k < 4 * NQ; ++k) P[k] = 0u;
+        // 4x4 byte transposes: dword q of column t holds rows 4q .. 4q+3 of that column.
+        uint32_t col[4][NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const uint32_t A = P[4 * q], B = P[4 * q + 1], C = P[4 * q + 2], Dd = P[4 * q + 3];
+            const uint32_t abLo = __builtin_amdgcn_perm(B, A, 0x05010400u);
+            const uint32_t abHi = __builtin_amdgcn_perm(B, A, 0x07030602u);
+            const uint32_t cdLo = __builtin_amdgcn_perm(Dd, C, 0x05010400u);
+            const uint32_t cdHi = __builtin_amdgcn_perm(Dd, C, 0x07030602u);
+            col[0][q] = __builtin_amdgcn_perm(cdLo, abLo, 0x05040100u);
+            col[1][q] = __builtin_amdgcn_perm(cdLo, abLo, 0x07060302u);
+            col[2][q] = __builtin_amdgcn_perm(cdHi, abHi, 0x05040100u);
+            col[3][q] = __builtin_amdgcn_perm(cdHi, abHi, 0x07060302u);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            if (c0 + t < n) {
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) col0[t * cstep + q] = col[t][q];
+            }
         }
     }
 }
@@ -446,12 +549,16 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
     // (sad_div_entry); a window without valid columns (s = 0) reads 255.
     u32x2* const div = reinterpret_cast<u32x2*>(rt + 4 * PD + ((sad_lcol(TL - 1, NQ) + NQ) & 1));
     if (threadIdx.x < 8) div[threadIdx.x] = sad_div_entry(threadIdx.x * nrows);
-    // Up to 768 columns (Dp up to 448 with R = 3) in one pass.
-    sad_stage<R, 3>(j, y, TL, TR, x0 - R, x0 - R - m - (Dp - 1), cap, [&](int c) -> uint32_t* {
-        if (c < TL) return lt + sad_lcol(c, NQ);
-        const int k = c - TL;
-        return rt + (k & 3) * PD + NQ * (k >> 2);
-    });
+    // Segments away from the image borders stage by column groups (at most 148 groups of four,
+    // for Dp = 64 or 512 with R = 3: one pass), the others column by column (up to 768 columns,
+    // Dp up to 448 with R = 3, in one pass). The choice is the block's, so no wave runs both.
+    const int lbase = x0 - R, rbase = x0 - R - m - (Dp - 1);
+    const bool aligned =
+        ((reinterpret_cast<uintptr_t>(j.left) | reinterpret_cast<uintptr_t>(j.right)) & 3u) == 0;
+    if (aligned && sad_groups_inside(TL, TR, lbase, rbase, W))
+        sad_stage_groups<R>(j, y, TL, TR, lbase, rbase, cap, lt, rt, PD);
+    else
+        sad_stage_columns<R, 3>(j, y, TL, TR, lbase, rbase, cap, lt, rt, PD);
     __syncthreads();
     const int quads = Dp / 4, tid = threadIdx.x;
     const int q = tid % quads, part = tid / quads;
@@ -499,6 +606,7 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
 #pragma unroll
     for (int k = 0; k < 3; ++k) rcol(2 - k, rr[k]);
     const int xf = x0 + i0;
+    const int tLo = m + dq - (xf - R), tHi = W - (xf - R);
     const uint32_t nwin = static_cast<uint32_t>(NW * nrows);
     const uint32_t magic = ((1u << 24) + nwin - 1) / nwin;
     // The checked form is right for every run; a wave takes it as a whole when any of its runs
@@ -518,7 +626,9 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
         rcol(3 + t, rr[0]);
         uint32_t lv[NQ];
         lcol(t, lv);
-        const int gc = xf - R + t;  // image column of left column t
+        // Left column t (image column xf - R + t) pairs with right image column
+        // xf - R + t - m - dq - k for disparity dq + k: a valid tap when t - k >= tLo and
+        // t < tHi (t - k is a constant in the unrolled runs).
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (checked) {
@@ -526,7 +636,7 @@ __device__ __forceinline__ void sad_cost_runs(const PairJob& j, int y, int x0, u
 #pragma unroll
                 for (int e = 0; e < NQ; ++e) a = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], a);
                 // Both tests evaluated (no short circuit), so the select stays a select.
-                acc[k] = (gc >= m + dq + k) & (gc < W) ? a : acc[k];
+                acc[k] = ((t - k >= tLo) & (t < tHi)) ? a : acc[k];
             } else {
 #pragma unroll
                 for (int e = 0; e < NQ; ++e) acc[k] = __builtin_amdgcn_sad_u8(lv[e], rr[k][e], acc[k]);
