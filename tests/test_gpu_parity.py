@@ -44,6 +44,9 @@ CASES = [
     (96, 64, 0, 48, 0, 4, {"preFilterCap": 1000, "blockSize": 5}),
     (96, 64, 0, 48, 0, 4, {"preFilterCap": 200, "blockSize": 9}),
     (70, 50, 0, 16, 0, 4, {"blockSize": -3}),    # empty window: every cost 255 (fill kernel)
+    # Wide enough for SAD segments away from the image borders (staged by column groups).
+    (700, 40, 0, 64, 0, 4, {"blockSize": 3}),
+    (700, 40, 0, 64, 0, 8, {"blockSize": 7, "preFilterCap": 200}),
 ]
 
 
@@ -121,6 +124,27 @@ def test_compute_device_buffers(gpu_available):
         got = do.cpu().numpy()
         np.testing.assert_array_equal(got, oracle.sgbm(p, left, right))
         np.testing.assert_array_equal(m.dump("output"), got)
+
+
+def test_compute_device_unaligned_images(gpu_available):
+    """Device images that do not start on a 4-byte boundary (views into larger buffers) give
+    the output of aligned copies: the SAD cost then stages every segment column by column."""
+    import torch
+    W, H = 700, 48
+    left, right = ls.synth_pair(W, H, 2, 60, seed=9)
+    for bs in (3, 5, 7):
+        p = _params(0, 64, 0, 4, {"blockSize": bs})
+        m = ls.StereoSGBM(p)
+        want = m.compute(left, right)
+        for off in (1, 2, 3):
+            bl = torch.zeros(W * H + 8, dtype=torch.uint8, device="cuda")
+            br = torch.zeros(W * H + 8, dtype=torch.uint8, device="cuda")
+            bl[off:off + W * H] = torch.from_numpy(left.reshape(-1)).cuda()
+            br[off:off + W * H] = torch.from_numpy(right.reshape(-1)).cuda()
+            do = torch.empty((H, W), dtype=torch.int16, device="cuda")
+            torch.cuda.synchronize()
+            m.compute_device(bl.data_ptr() + off, br.data_ptr() + off, W, H, do.data_ptr(), sync=True)
+            np.testing.assert_array_equal(do.cpu().numpy(), want)
 
 
 def test_bgr_input_large(gpu_available):
