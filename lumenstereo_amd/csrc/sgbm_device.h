@@ -927,8 +927,12 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 // every access instruction of a row covers contiguous bytes whatever K is. For K = 4, 6 and 8
 // there is one chunk and lane l simply holds d = lK .. lK+K-1.
 // LPS = 32: for K = 2, 4, 8 or 16 one chunk, lane l holding d = lK .. lK+K-1; otherwise
-// chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities laid out as above with 32
-// lanes, so each access of a scanline covers 64 to 512 contiguous bytes.
+// chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities, the 32 lanes of a chunk again
+// covering 32Q consecutive disparities, so each access of a scanline covers 64 to 512
+// contiguous bytes. The chunks alternate in direction (snake_chunks): even chunks run up the
+// lanes (lane l holds 32Qc + Ql ..), odd chunks down them (lane l holds 32Qc + Q(31-l) ..), so
+// the disparities either side of a chunk boundary sit in the same lane (lane 31 after an even
+// chunk, lane 0 after an odd one) and no value crosses between the two halves of the wave.
 // Registers hold the lane's elements chunk by chunk, element e being slice_d<K, LPS>(l, e).
 template <int K, int LPS = 16>
 __host__ __device__ constexpr int slice_q() {
@@ -937,9 +941,23 @@ __host__ __device__ constexpr int slice_q() {
 }
 
 template <int K, int LPS = 16>
+__host__ __device__ constexpr bool snake_chunks() {
+    return LPS == 32 && K / slice_q<K, LPS>() > 1;
+}
+
+template <int K, int LPS = 16>
 __device__ __forceinline__ int slice_d(int l, int e) {
     constexpr int Q = slice_q<K, LPS>();
-    return LPS * Q * (e / Q) + Q * l + e % Q;
+    const int c = e / Q;
+    const int ll = snake_chunks<K, LPS>() && (c & 1) ? LPS - 1 - l : l;
+    return LPS * Q * c + Q * ll + e % Q;
+}
+
+// Elements from lane l's position (Q*l) to its position in an odd chunk of a snake layout.
+template <int K, int LPS = 16>
+__device__ __forceinline__ int odd_chunk_offset(int l) {
+    constexpr int Q = slice_q<K, LPS>();
+    return snake_chunks<K, LPS>() ? Q * (LPS - 1 - 2 * l) : 0;
 }
 
 // Lane that holds disparity d, and its element index there (16-lane rows).
@@ -967,14 +985,14 @@ struct CostBytes {
     uint32_t w[cost_pairs<K, LPS>() ? K / 2 : K / 4];
 };
 
-// p: the pixel's cost vector + Q*l (the lane's first chunk).
+// p: the pixel's cost vector + Q*l (the lane's first chunk); odd: odd_chunk_offset(l).
 template <int K, int LPS = 16>
-__device__ __forceinline__ CostBytes<K, LPS> load_cost(const LUMEN_G uint8_t* p) {
+__device__ __forceinline__ CostBytes<K, LPS> load_cost(const LUMEN_G uint8_t* p, int odd = 0) {
     constexpr int Q = slice_q<K, LPS>();
     CostBytes<K, LPS> c;
 #pragma unroll
     for (int ch = 0; ch < K / Q; ++ch) {
-        const LUMEN_G uint8_t* q = p + LPS * Q * ch;
+        const LUMEN_G uint8_t* q = p + LPS * Q * ch + (snake_chunks<K, LPS>() && (ch & 1) ? odd : 0);
         if constexpr (Q == 2) {
             c.w[ch] = *reinterpret_cast<const LUMEN_G uint16_t*>(q);
         } else if constexpr (Q == 6) {
@@ -1006,14 +1024,14 @@ __device__ __forceinline__ CostBytes<K, LPS> load_cost(const LUMEN_G uint8_t* p)
     return c;
 }
 
-// p: the pixel's path-cost vector + Q*l. Each chunk store writes a whole contiguous run of
-// the scanline's lanes, so the stores are nontemporal.
+// p: the pixel's path-cost vector + Q*l; odd: odd_chunk_offset(l). Each chunk store writes a
+// whole contiguous run of the scanline's lanes, so the stores are nontemporal.
 template <int K, int LPS = 16>
-__device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)[K / 2]) {
+__device__ __forceinline__ void store_path(LUMEN_G uint16_t* p, const u16x2 (&v)[K / 2], int odd = 0) {
     constexpr int Q = slice_q<K, LPS>();
 #pragma unroll
     for (int ch = 0; ch < K / Q; ++ch) {
-        LUMEN_G uint16_t* q = p + LPS * Q * ch;
+        LUMEN_G uint16_t* q = p + LPS * Q * ch + (snake_chunks<K, LPS>() && (ch & 1) ? odd : 0);
         const int b = ch * (Q / 2);
         if constexpr (Q == 2) {
             __builtin_nontemporal_store(bits(v[b]), reinterpret_cast<LUMEN_G uint32_t*>(q));
@@ -1050,52 +1068,60 @@ __device__ __forceinline__ void expand_cost(const CostBytes<K, LPS>& c, u16x2 cs
     }
 }
 
-// The value lane i + 32 (for i < 32) or i - 32 holds: the other half of the wave.
-__device__ __forceinline__ uint32_t other_half(uint32_t x, bool upper) {
-    const auto sw = __builtin_amdgcn_permlane32_swap(x, x, false, false);
-    return upper ? static_cast<uint32_t>(sw[0]) : static_cast<uint32_t>(sw[1]);
-}
-
 // d-1 and d+1 neighbour vectors of a packed slice (lane l of its scanline's lanes). Inside a
-// chunk they are the lane's own neighbouring elements. At a chunk's start, d-1 is the last
-// element of the same chunk of lane l-1, or for lane 0 the last element of the previous chunk
-// of the scanline's last lane; at a chunk's end, d+1 is the first element of the same chunk of
-// lane l+1, or for the last lane of the next chunk of lane 0. 16-lane rows: one row rotation
-// per chunk each way, then a lane select. 32-lane scanlines: whole-wave rotations by one lane
-// (which cross the boundary between the two DPP rows of a scanline); the wrap from a
-// scanline's last lane to its lane 0 is the rotation's value in the other half of the wave
-// (lane 32 holds lane 31's, lane 0 lane 63's), fetched with v_permlane32_swap. The values lane
-// 0 gets for d = -1 and the last lane for d = Dp are masked by the caller.
+// chunk they are the lane's own neighbouring elements. At a chunk's first element, d-1 is the
+// last element of the same chunk in the lane before it in the chunk's direction; at a chunk's
+// last element, d+1 is the first element of the same chunk in the lane after it. At a chunk's
+// edge lanes the neighbour lies in the adjacent chunk: 16-lane rows (one row rotation per
+// chunk each way) take it from the rotated value of that chunk (lane 0's d-1 is the last lane's
+// last element of the previous chunk); 32-lane scanlines (whole-wave rotations by one lane,
+// which cross the boundary between the two DPP rows of a scanline) alternate the chunks'
+// direction (snake_chunks), so there the neighbour is the lane's own element of the adjacent
+// chunk. The values the lowest lane gets for d = -1 and the highest for d = Dp (and any value
+// rotated in from the other scanline of a 32-lane wave, which only reaches those lanes) are
+// masked by the caller.
 template <int K, int LPS = 16>
 __device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&dm1)[K / 2],
                                            uint32_t (&dp1)[K / 2], int l) {
     constexpr int Q = slice_q<K, LPS>(), NC = K / Q, PQ = Q / 2;
-    uint32_t prevLast[NC], nextFirst[NC], prevWrap[NC], nextWrap[NC];
-    const bool upper = (threadIdx.x & 63) >= 32;
+    uint32_t fromPrev[NC], fromNext[NC];
+    if constexpr (LPS == 32) {
+        static_assert(NC == 1 || snake_chunks<K, LPS>(), "multi-chunk 32-lane slices alternate");
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        if constexpr (LPS == 32) {
-            prevLast[c] = dpp<kWaveRor1>(bits(v[c * PQ + PQ - 1]));
-            nextFirst[c] = dpp<kWaveRol1>(bits(v[c * PQ]));
-            if (c < NC - 1) prevWrap[c] = other_half(prevLast[c], upper);
-            if (c > 0) nextWrap[c] = other_half(nextFirst[c], upper);
-        } else {
+        for (int c = 0; c < NC; ++c) {
+            const bool odd = (c & 1) != 0;
+            const uint32_t lastW = bits(v[c * PQ + PQ - 1]), firstW = bits(v[c * PQ]);
+            // Even chunks run up the lanes (d-1 from lane l-1), odd ones down (from lane l+1).
+            uint32_t pv = odd ? dpp<kWaveRol1>(lastW) : dpp<kWaveRor1>(lastW);
+            uint32_t nv = odd ? dpp<kWaveRor1>(firstW) : dpp<kWaveRol1>(firstW);
+            const int lowLane = odd ? LPS - 1 : 0;   // holds the chunk's lowest disparities
+            const int highLane = odd ? 0 : LPS - 1;  // holds its highest
+            if (c > 0) pv = l == lowLane ? bits(v[(c - 1) * PQ + PQ - 1]) : pv;
+            if (c < NC - 1) nv = l == highLane ? bits(v[(c + 1) * PQ]) : nv;
+            fromPrev[c] = pv;
+            fromNext[c] = nv;
+        }
+    } else {
+        uint32_t prevLast[NC], nextFirst[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
             prevLast[c] = dpp<kRowRor1>(bits(v[c * PQ + PQ - 1]));
             nextFirst[c] = dpp<kRowRor15>(bits(v[c * PQ]));
-            prevWrap[c] = prevLast[c];
-            nextWrap[c] = nextFirst[c];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            fromPrev[c] = (c > 0 && l == 0) ? prevLast[c - 1] : prevLast[c];
+            fromNext[c] = (c < NC - 1 && l == LPS - 1) ? nextFirst[c + 1] : nextFirst[c];
         }
     }
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-        const uint32_t fromPrev = (c > 0 && l == 0) ? prevWrap[c - 1] : prevLast[c];
-        const uint32_t fromNext = (c < NC - 1 && l == LPS - 1) ? nextWrap[c + 1] : nextFirst[c];
 #pragma unroll
         for (int jj = 0; jj < PQ; ++jj) {
             const int q = c * PQ + jj;
             const uint32_t cur = bits(v[q]);
-            const uint32_t lo = jj == 0 ? fromPrev : bits(v[q - 1]);
-            const uint32_t hi = jj == PQ - 1 ? fromNext : bits(v[q + 1]);
+            const uint32_t lo = jj == 0 ? fromPrev[c] : bits(v[q - 1]);
+            const uint32_t hi = jj == PQ - 1 ? fromNext[c] : bits(v[q + 1]);
             dm1[q] = (cur << 16) | (lo >> 16);
             dp1[q] = (cur >> 16) | (hi << 16);
         }
@@ -1183,6 +1209,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     // The padding Dp - D is under 64, or under 32 for Dp = 96 (padded_range).
     static_assert(LPS * Q >= 64, "padding must fit in the last chunk");
     const LUMEN_G uint8_t* cbase = gmem(j.cost) + l * Q;
+    const int odd = odd_chunk_offset<K, LPS>(l);  // to the lane's place in odd chunks
     LUMEN_G uint16_t* Lbase = gmem(j.L) + (size_t)path * j.pathStride + l * Q;
     const u16x2 cs = splat(static_cast<uint32_t>(j.costScale));
     const u16x2 P1p = splat(static_cast<uint32_t>(j.P1));
@@ -1204,13 +1231,13 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     LUMEN_G uint16_t* const trash = gmem(trash_base) + g * LPS * K + l * Q;
     u16x2 Lv[K / 2];
     {
-        const CostBytes<K, LPS> c0 = load_cost<K, LPS>(lp);
+        const CostBytes<K, LPS> c0 = load_cost<K, LPS>(lp, odd);
         expand_cost<K, LPS>(c0, cs, Lv);
         if constexpr (PAD) {
 #pragma unroll
             for (int q = 0; q < K / 2; ++q) Lv[q] = as16(bits(Lv[q]) | padm[q]);
         }
-        store_path<K, LPS>(len > 0 ? sp : trash, Lv);
+        store_path<K, LPS>(len > 0 ? sp : trash, Lv, odd);
     }
     uint32_t M = group_min<LPS>(slice_min<K>(Lv));
 
@@ -1235,7 +1262,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
                 const bool adv = lt < last;
                 lp = adv ? lp + cstep : lp;
                 lt += adv ? 1 : 0;
-                dst[u] = load_cost<K, LPS>(lp);
+                dst[u] = load_cost<K, LPS>(lp, odd);
             }
         } else {
             const int ti = tb + lu;  // >= 1 and <= last
@@ -1249,7 +1276,7 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
                 // u of the block in registers, 2*PF registers that cost waves per SIMD. The
                 // empty asm only makes the pointer opaque; it emits nothing.
                 if constexpr (HDIR == 0) asm volatile("" : "+v"(lp));
-                dst[u] = load_cost<K, LPS>(lp);
+                dst[u] = load_cost<K, LPS>(lp, odd);
             }
             lt += PF;
         }
@@ -1290,9 +1317,9 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
                 Lv[q] = nv;
             }
             if constexpr (decltype(checked)::value)
-                store_path<K, LPS>(tb + u < len ? sp : trash, Lv);
+                store_path<K, LPS>(tb + u < len ? sp : trash, Lv, odd);
             else
-                store_path<K, LPS>(sp, Lv);
+                store_path<K, LPS>(sp, Lv, odd);
             M = group_min<LPS>(slice_min<K>(Lv));
         }
     };
