@@ -173,7 +173,9 @@ typedef struct lumen_schedule {
                               half the disparities per lane: a shorter dependency chain per
                               step for twice the waves) instead of 16 (four per wave): 0 none,
                               1 the straight pair along the longer image dimension, 2 all four
-                              straight paths, 3 every path. */
+                              straight paths, 3 every path; 4, 5 and 6 the same sets on 64-lane
+                              scanlines (one per wave, a quarter of the disparities per lane;
+                              padded ranges that are a multiple of 128 only). */
     int32_t agg_occupancy; /* aggregation blocks (of four waves) per CU at most: 0 no cap, or
                               2, 3 or 4 (the kernel then reserves LDS it does not use). Fewer
                               concurrent scanlines per CU can move a large frame's bytes

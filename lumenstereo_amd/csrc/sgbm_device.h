@@ -917,8 +917,9 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
     step = dy * W + dx;
 }
 
-// Slice layout. A scanline's disparity vector is spread over LPS lanes (a 16-lane DPP row, or
-// 32 lanes: two rows), lane l holding K of the pixel's Dp = LPS*K disparities.
+// Slice layout. A scanline's disparity vector is spread over LPS lanes (a 16-lane DPP row, 32
+// lanes: two rows, or the whole 64-lane wave), lane l holding K of the pixel's Dp = LPS*K
+// disparities.
 //
 // LPS = 16: K/Q chunks of Q consecutive disparities (Q = 8 when K is a multiple of 8, 4 when
 // it is a multiple of 4, else K: K = 6, Dp = 96): chunk c of lane l is d = 16Qc + Ql ..
@@ -926,23 +927,23 @@ __device__ __forceinline__ void scan_geom(int path, int s, int W, int H, int& p0
 // 8-, 12- or 16-byte path-cost access per lane, and the 16 lanes' chunks are contiguous, so
 // every access instruction of a row covers contiguous bytes whatever K is. For K = 4, 6 and 8
 // there is one chunk and lane l simply holds d = lK .. lK+K-1.
-// LPS = 32: for K = 2, 4, 8 or 16 one chunk, lane l holding d = lK .. lK+K-1; otherwise
-// chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities, the 32 lanes of a chunk again
-// covering 32Q consecutive disparities, so each access of a scanline covers 64 to 512
+// LPS = 32 or 64: for K = 2, 4, 8 or 16 one chunk, lane l holding d = lK .. lK+K-1; otherwise
+// chunks of Q = 4 (K = 12) or Q = 2 (K = 6, 10, 14) disparities, the LPS lanes of a chunk again
+// covering LPS*Q consecutive disparities, so each access of a scanline covers 64 to 1024
 // contiguous bytes. The chunks alternate in direction (snake_chunks): even chunks run up the
-// lanes (lane l holds 32Qc + Ql ..), odd chunks down them (lane l holds 32Qc + Q(31-l) ..), so
-// the disparities either side of a chunk boundary sit in the same lane (lane 31 after an even
-// chunk, lane 0 after an odd one) and no value crosses between the two halves of the wave.
+// lanes (lane l holds LPS*Qc + Ql ..), odd chunks down them (lane l holds LPS*Qc + Q(LPS-1-l)
+// ..), so the disparities either side of a chunk boundary sit in the same lane (the last lane
+// after an even chunk, lane 0 after an odd one) and no value crosses between DPP rows there.
 // Registers hold the lane's elements chunk by chunk, element e being slice_d<K, LPS>(l, e).
 template <int K, int LPS = 16>
 __host__ __device__ constexpr int slice_q() {
-    return LPS == 32 ? ((K == 2 || K == 4 || K == 8 || K == 16) ? K : (K % 4 == 0 ? 4 : 2))
+    return LPS >= 32 ? ((K == 2 || K == 4 || K == 8 || K == 16) ? K : (K % 4 == 0 ? 4 : 2))
                      : (K % 8 == 0 ? 8 : K % 4 == 0 ? 4 : K);
 }
 
 template <int K, int LPS = 16>
 __host__ __device__ constexpr bool snake_chunks() {
-    return LPS == 32 && K / slice_q<K, LPS>() > 1;
+    return LPS >= 32 && K / slice_q<K, LPS>() > 1;
 }
 
 template <int K, int LPS = 16>
@@ -1074,8 +1075,8 @@ __device__ __forceinline__ void expand_cost(const CostBytes<K, LPS>& c, u16x2 cs
 // last element, d+1 is the first element of the same chunk in the lane after it. At a chunk's
 // edge lanes the neighbour lies in the adjacent chunk: 16-lane rows (one row rotation per
 // chunk each way) take it from the rotated value of that chunk (lane 0's d-1 is the last lane's
-// last element of the previous chunk); 32-lane scanlines (whole-wave rotations by one lane,
-// which cross the boundary between the two DPP rows of a scanline) alternate the chunks'
+// last element of the previous chunk); 32- and 64-lane scanlines (whole-wave rotations by one
+// lane, which cross the boundaries between the DPP rows of a scanline) alternate the chunks'
 // direction (snake_chunks), so there the neighbour is the lane's own element of the adjacent
 // chunk. The values the lowest lane gets for d = -1 and the highest for d = Dp (and any value
 // rotated in from the other scanline of a 32-lane wave, which only reaches those lanes) are
@@ -1085,8 +1086,8 @@ __device__ __forceinline__ void neighbours(const u16x2 (&v)[K / 2], uint32_t (&d
                                            uint32_t (&dp1)[K / 2], int l) {
     constexpr int Q = slice_q<K, LPS>(), NC = K / Q, PQ = Q / 2;
     uint32_t fromPrev[NC], fromNext[NC];
-    if constexpr (LPS == 32) {
-        static_assert(NC == 1 || snake_chunks<K, LPS>(), "multi-chunk 32-lane slices alternate");
+    if constexpr (LPS >= 32) {
+        static_assert(NC == 1 || snake_chunks<K, LPS>(), "multi-chunk wide slices alternate");
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const bool odd = (c & 1) != 0;
@@ -1137,14 +1138,19 @@ __device__ __forceinline__ uint32_t slice_min(const u16x2 (&v)[K / 2]) {
     return min(b & 0xFFFFu, b >> 16);
 }
 
-// Minimum over the LPS lanes of a scanline, left in each of them: the DPP row minimum, and
-// for 32 lanes the minimum with the other row of the pair (v_permlane16_swap exchanges rows 0
-// and 1, 2 and 3).
+// Minimum over the LPS lanes of a scanline, left in each of them: the DPP row minimum; for 32
+// or 64 lanes the minimum with the other row of the pair (v_permlane16_swap exchanges rows 0
+// and 1, 2 and 3); for 64 lanes then with the other pair (v_permlane32_swap exchanges the
+// wave's halves).
 template <int LPS>
 __device__ __forceinline__ uint32_t group_min(uint32_t v) {
     v = row_min(v);
-    if constexpr (LPS == 32) {
+    if constexpr (LPS >= 32) {
         const auto sw = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        v = min(static_cast<uint32_t>(sw[0]), static_cast<uint32_t>(sw[1]));
+    }
+    if constexpr (LPS == 64) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(v, v, false, false);
         v = min(static_cast<uint32_t>(sw[0]), static_cast<uint32_t>(sw[1]));
     }
     return v;
@@ -1185,7 +1191,7 @@ __device__ __forceinline__ uint32_t p2_lookup(const P2Table& t, uint32_t g) {
 // offsets instead of a 64-bit add each. HDIR = 0 serves every path with the step at run time.
 template <int K, bool PAD, int PF = default_prefetch(K), int LPS = 16, int HDIR = 0>
 __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_base) {
-    static_assert(LPS == 16 || LPS == 32, "a scanline spans one or two DPP rows");
+    static_assert(LPS == 16 || LPS == 32 || LPS == 64, "a scanline spans one, two or four DPP rows");
     static_assert(K % 2 == 0, "a lane holds whole packed pairs of disparities");
     static_assert(valid_prefetch(K, PF, LPS), "unsupported prefetch depth for this slice");
     static_assert(HDIR >= -1 && HDIR <= 1, "HDIR: 0 run-time step, +1 LR, -1 RL");
@@ -1244,8 +1250,8 @@ __device__ void paths_wave(const PairJob& j, int path, int s0, uint16_t* trash_b
     // Two register blocks of PF steps alternate: while one block is computed, the costs of the
     // next are in flight. Lane l of a scanline also fetches the two image bytes of step
     // tb + (l % PF) and looks up t16(P2adj) once per block; each step broadcasts its lane's
-    // value with DPP row_newbcast (PF <= 16, so both rows of a 32-lane scanline hold the same
-    // values). CHECKED blocks handle scanline ends: loads past the end re-read the last pixel,
+    // value with DPP row_newbcast (PF <= 16, so every row of a 32- or 64-lane scanline holds the
+    // same values). CHECKED blocks handle scanline ends: loads past the end re-read the last pixel,
     // and steps past the end are computed but stored to the trash slot, so every store is
     // unconditional and the wait counts stay exact. While every scanline of the wave has all of
     // a block's loads and stores inside it, the blocks skip those clamps (no per-step compares

@@ -24,11 +24,12 @@ constexpr int kTrashElems = 64 * 32;
 // last chunk.
 inline int padded_range(int D) { return D > 64 && D <= 96 ? 96 : (D + 63) / 64 * 64; }
 
-// Vector stores per aggregation step and lane. 16-lane scanlines: K/8 16-byte stores, K/4
-// 8-byte ones, or (K = 6) one 12-byte one. 32-lane scanlines: one 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones
-// (K = 16), K/4 8-byte ones (K = 12) or K/2 4-byte ones (K = 6, 10, 14).
+// Vector stores per aggregation step and lane (K: entries per lane). 16-lane scanlines: K/8
+// 16-byte stores, K/4 8-byte ones, or (K = 6) one 12-byte one. 32- and 64-lane scanlines: one
+// 4-, 8- or 16-byte store (K = 2, 4, 8), two 16-byte ones (K = 16), K/4 8-byte ones (K = 12) or
+// K/2 4-byte ones (K = 6, 10, 14).
 __host__ __device__ constexpr int path_stores(int K, int LPS = 16) {
-    return LPS == 32 ? (K == 2 || K == 4 || K == 8 ? 1 : K == 16 ? 2 : K % 4 == 0 ? K / 4 : K / 2)
+    return LPS >= 32 ? (K == 2 || K == 4 || K == 8 ? 1 : K == 16 ? 2 : K % 4 == 0 ? K / 4 : K / 2)
                      : (K % 8 == 0 ? K / 8 : K % 4 == 0 ? K / 4 : 1);
 }
 
@@ -46,6 +47,11 @@ __host__ __device__ constexpr bool wide_scanlines_ok(int K, int PF) {
     return K % 4 == 0 && valid_prefetch(K / 2, PF, 32);
 }
 
+// The same for 64-lane scanlines (one per wave, K/4 disparities per lane): Dp a multiple of 128.
+__host__ __device__ constexpr bool xwide_scanlines_ok(int K, int PF) {
+    return K % 8 == 0 && valid_prefetch(K / 4, PF, 64);
+}
+
 __host__ __device__ constexpr int default_prefetch(int K) { return K <= 8 ? 8 : 4; }
 
 // Schedule of one pair: how its work is grouped and ordered, never what is computed. Every
@@ -56,7 +62,8 @@ struct Schedule {
     int censusTile;   // census cost segment width: 128, 256 or 512 pixels
     int aggWide;      // paths on 32-lane scanlines (two per wave): 0 none, 1 the straight
                       // pair along the longer image dimension, 2 all four straight paths,
-                      // 3 every path; the others run 16-lane ones
+                      // 3 every path; 4, 5, 6 the same sets on 64-lane scanlines (one per
+                      // wave); the others run 16-lane ones
     int aggOccupancy; // aggregation blocks per CU at most: 0 no cap, 2, 3 or 4
 };
 
@@ -97,7 +104,8 @@ struct StageEvents {
 // path r's scanlines.
 struct PathsGrid {
     int waveStart[kMaxPaths + 1];
-    int wideMask;  // bit r: path r runs 32-lane scanlines
+    int wideMask;   // bit r: path r runs 32-lane scanlines
+    int xwideMask;  // bit r: path r runs 64-lane scanlines
 };
 
 // The aggregation launch of slice K (sgbm_paths_impl.h; instantiated in sgbm_paths_k*.hip).

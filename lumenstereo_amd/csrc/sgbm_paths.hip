@@ -10,9 +10,10 @@ namespace lumen {
 
 int scanline_count(int path, int W, int H) { return scan_count(path, W, H); }
 
-// Paths that run 32-lane scanlines under aggregation lane setting `wide` (Schedule::aggWide).
+// Paths that run wider scanlines (32 lanes for 1-3, 64 for 4-6) under aggregation lane setting
+// `wide` (Schedule::aggWide).
 int wide_path_mask(int wide, int W, int H, int npaths) {
-    switch (wide) {
+    switch (wide > 3 ? wide - 3 : wide) {
         case 1: return W >= H ? 0x3 : 0xC;  // the straight pair along the longer dimension
         case 2: return 0xF;                 // all four straight paths
         case 3: return (1 << npaths) - 1;   // every path
@@ -49,13 +50,17 @@ static hipError_t occupancy_lds(int n, size_t& lds) {
 
 hipError_t launch_paths(const PairJob& j, uint16_t* trash, hipStream_t s, StageEvents ev) {
     PathsGrid g;
-    if (j.sched.aggWide != 0 && !wide_scanlines_ok(j.K, j.sched.aggPrefetch))
+    const int w = j.sched.aggWide;
+    if (w < 0 || w > 6 || (w >= 1 && w <= 3 && !wide_scanlines_ok(j.K, j.sched.aggPrefetch)) ||
+        (w >= 4 && !xwide_scanlines_ok(j.K, j.sched.aggPrefetch)))
         return hipErrorInvalidValue;
-    g.wideMask = wide_path_mask(j.sched.aggWide, j.W, j.H, j.npaths);
+    const int mask = wide_path_mask(w, j.W, j.H, j.npaths);
+    g.wideMask = w <= 3 ? mask : 0;
+    g.xwideMask = w >= 4 ? mask : 0;
     int waves = 0;
     for (int r = 0; r < j.npaths; ++r) {
         g.waveStart[r] = waves;
-        const int per = (g.wideMask >> r) & 1 ? 2 : 4;
+        const int per = (g.xwideMask >> r) & 1 ? 1 : (g.wideMask >> r) & 1 ? 2 : 4;
         waves += (scan_count(r, j.W, j.H) + per - 1) / per;
     }
     for (int r = j.npaths; r <= kMaxPaths; ++r) g.waveStart[r] = waves;

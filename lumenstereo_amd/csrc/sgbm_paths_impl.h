@@ -12,7 +12,8 @@ namespace lumen {
 // Wave w takes the next scanlines of the path whose range of waves contains w (paths in order,
 // so the long horizontal scanlines start first): four of 16 lanes each, or, for the paths in
 // wideMask, two of 32 lanes each (half the disparities per lane, so about a third fewer
-// instructions per step on each scanline's dependency chain, for twice the waves).
+// instructions per step on each scanline's dependency chain, for twice the waves), or for those
+// in xwideMask one of 64 lanes (a quarter of the disparities per lane).
 template <int K, bool PAD, int PF>
 __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, uint16_t* trash) {
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -21,8 +22,8 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, 
 #pragma unroll
     for (int r = 1; r < kMaxPaths; ++r)
         if (r < j.npaths && wave >= g.waveStart[r]) path = r;
-    const bool wide = (g.wideMask >> path) & 1;
-    const int s0 = (wave - g.waveStart[path]) * (wide ? 2 : 4);
+    const bool wide = (g.wideMask >> path) & 1, xwide = (g.xwideMask >> path) & 1;
+    const int s0 = (wave - g.waveStart[path]) * (xwide ? 1 : wide ? 2 : 4);
     // The launch ends when its longest scanlines end. Priority 1: the longer straight path
     // first, then the other straight path, diagonals last; priority 2: by scanline length.
     if (j.sched.aggPriority == 1) {
@@ -37,6 +38,17 @@ __global__ __launch_bounds__(256) void sgm_paths_kernel(PairJob j, PathsGrid g, 
         set_priority_for_length(scan_length(path, s0, j.W, j.H));
     }
     // The horizontal paths run with their step known at compile time (immediate offsets).
+    if constexpr (xwide_scanlines_ok(K, PF)) {
+        if (xwide) {
+            if (path == 0)
+                paths_wave<K / 4, PAD, PF, 64, 1>(j, path, s0, trash);
+            else if (path == 1)
+                paths_wave<K / 4, PAD, PF, 64, -1>(j, path, s0, trash);
+            else
+                paths_wave<K / 4, PAD, PF, 64>(j, path, s0, trash);
+            return;
+        }
+    }
     if constexpr (wide_scanlines_ok(K, PF)) {
         if (wide) {
             if (path == 0)

@@ -145,18 +145,26 @@ bool valid_schedule(const Schedule& s, int K, std::string& why) {
         why = "census tile must be 128, 256 or 512";
         return false;
     }
-    if (s.aggWide < 0 || s.aggWide > 3) {
-        why = "aggregation lane setting must be 0, 1, 2 or 3";
+    if (s.aggWide < 0 || s.aggWide > 6) {
+        why = "aggregation lane setting must be 0 to 6";
         return false;
     }
     if (s.aggOccupancy != 0 && (s.aggOccupancy < 2 || s.aggOccupancy > 4)) {
         why = "aggregation occupancy must be 0, 2, 3 or 4 blocks per CU";
         return false;
     }
-    if (s.aggWide != 0 && !wide_scanlines_ok(K, s.aggPrefetch)) {
+    if (s.aggWide >= 1 && s.aggWide <= 3 && !wide_scanlines_ok(K, s.aggPrefetch)) {
         why = "aggregation prefetch " + std::to_string(s.aggPrefetch) +
               " is not supported on 32-lane scanlines for " + std::to_string(K * kScanLanes) +
               " padded disparities";
+        return false;
+    }
+    if (s.aggWide >= 4 && !xwide_scanlines_ok(K, s.aggPrefetch)) {
+        why = K % 8 != 0 ? "64-lane scanlines need a padded range that is a multiple of 128, not " +
+                               std::to_string(K * kScanLanes)
+                         : "aggregation prefetch " + std::to_string(s.aggPrefetch) +
+                               " is not supported on 64-lane scanlines for " +
+                               std::to_string(K * kScanLanes) + " padded disparities";
         return false;
     }
     return true;

@@ -10,7 +10,8 @@ and ordered on the device, never what is computed:
 * ``agg_wide``: which aggregation paths run scanlines of 32 lanes (two per wave, half the
   disparities per lane: fewer instructions on each step's dependency chain, twice the waves)
   instead of 16 (four per wave): none, the straight pair along the longer image dimension, all
-  straight paths, or every path;
+  straight paths, or every path (1-3); 4-6 name the same sets on 64-lane scanlines (one per
+  wave, a quarter of the disparities per lane) where the padded range is a multiple of 128;
 * ``agg_occupancy``: at most 2, 3 or 4 aggregation blocks per CU, or no cap (0). A large frame
   can move its bytes faster with fewer scanlines in flight per CU.
 
@@ -59,11 +60,12 @@ def valid_prefetch(K: int, pf: int) -> bool:
 
 
 def wide_settings(K: int):
-    """agg_wide values a slice of K disparities per 16 lanes supports: every one when K is a
-    multiple of 4 (32-lane scanlines split the padded range into whole chunks of packed pairs,
-    and store no more per step than 16-lane ones, so every valid prefetch depth fits them too);
-    only 16-lane scanlines for K = 6 (Dp = 96: three disparities per lane of 32)."""
-    return (0, 1, 2, 3) if K % 4 == 0 else (0,)
+    """agg_wide values a slice of K disparities per 16 lanes supports: 0-3 when K is a multiple
+    of 4 (32-lane scanlines split the padded range into whole chunks of packed pairs, and store
+    no more per step than 16-lane ones, so every valid prefetch depth fits them too), 4-6 too
+    when K is a multiple of 8 (64-lane ones, K/4 disparities per lane); only 16-lane scanlines
+    for K = 6 (Dp = 96: three disparities per lane of 32)."""
+    return tuple(range(7)) if K % 8 == 0 else (0, 1, 2, 3) if K % 4 == 0 else (0,)
 
 
 def default_schedule(K: int) -> dict:

@@ -12,15 +12,16 @@ from lumenstereo_amd import tune
 pytestmark = pytest.mark.gpu
 
 CASES = [
-    # (W, H, D, census, paths); 32-lane scanlines hold K/2 disparities per lane
+    # (W, H, D, census, paths); 32-lane scanlines hold K/2 disparities per lane, 64-lane ones
+    # (K a multiple of 8) K/4
     (96, 64, 64, False, 4),     # K = 4 (32-lane: one chunk of 2)
-    (130, 70, 128, True, 8),    # K = 8 (32-lane: one chunk of 4)
+    (130, 70, 128, True, 8),    # K = 8 (32-lane: one chunk of 4; 64-lane: one chunk of 2)
     (120, 60, 170, True, 4),    # K = 12, padded (32-lane: three chunks of 2)
     (150, 50, 290, False, 4),   # K = 20, padded (32-lane: five chunks of 2)
-    (100, 40, 384, True, 8),    # K = 24 (32-lane: three chunks of 4)
+    (100, 40, 384, True, 8),    # K = 24 (32-lane: three chunks of 4; 64-lane: three of 2)
     (60, 90, 40, True, 8),      # K = 4 padded (40 -> 64), taller than wide
-    (100, 33, 250, False, 8),   # K = 16 padded (32-lane: one chunk of 8)
-    (70, 41, 512, True, 4),     # K = 32 (32-lane: one chunk of 16)
+    (100, 33, 250, False, 8),   # K = 16 padded (32-lane: one chunk of 8; 64-lane: one of 4)
+    (70, 41, 512, True, 4),     # K = 32 (32-lane: one chunk of 16; 64-lane: one of 8)
     (80, 45, 448, False, 8),    # K = 28 (32-lane: seven chunks of 2)
     (110, 60, 96, True, 8),     # K = 6 (Dp = 96: three chunks of 2 on 16 lanes; no 32-lane)
     (90, 70, 75, False, 4),     # K = 6 padded (75 -> 96)
@@ -58,11 +59,14 @@ def test_every_candidate_schedule_is_bit_identical(gpu_available, case):
                                   (90, 50, 0, 256, 1, 4, {}), (80, 30, 0, 512, 0, 8, {}),
                                   (90, 50, 0, 192, 1, 8, {}), (70, 40, 5, 295, 0, 4, {}),
                                   (60, 30, 0, 448, 1, 8, {}), (66, 35, 0, 384, 0, 4,
-                                                               {"P1": 30000, "P2": 60000})],
+                                                               {"P1": 30000, "P2": 60000}),
+                                  (100, 60, 0, 128, 1, 8, {}), (75, 44, 2, 102, 0, 8, {}),
+                                  (50, 80, 0, 230, 1, 4, {}), (64, 36, 1, 385, 0, 8, {})],
                          ids=lambda c: f"{c[0]}x{c[1]}_d{c[2]}-{c[3]}_m{c[4]}_p{c[5]}")
 def test_wide_scanlines_every_path_equals_oracle(gpu_available, case):
-    """Every path on 32-lane scanlines: each path's aggregated costs equal the oracle's (wide
-    scanlines, padded ranges, taller-than-wide frames, minDisparity > 0, forced uint16 wrap)."""
+    """Every path on 32-lane scanlines, and where the padded range is a multiple of 128 on
+    64-lane ones: each path's aggregated costs equal the oracle's (wide scanlines, padded
+    ranges, taller-than-wide frames, minDisparity > 0, forced uint16 wrap)."""
     W, H, minD, maxD, mode, paths, extra = case
     p = ls.StereoParams(minDisparity=minD, maxDisparity=maxD, numDirections=paths,
                         matchingCostMode=ls.MatchingCostMode(mode), uniquenessRatio=0,
@@ -72,7 +76,8 @@ def test_wide_scanlines_every_path_equals_oracle(gpu_available, case):
     left, right = ls.synth_pair(W, H, max(minD, 2), maxD - 4, seed=W * 31 + H)
     want, st = oracle.sgbm(p, left, right, dumps=True)
     m = ls.StereoSGBM(p)
-    for wide in (1, 2, 3):
+    K = tune.padded_range(maxD - minD) // 16
+    for wide in tune.wide_settings(K)[1:]:
         m.setSchedule({"agg_prefetch": 8, "agg_priority": 1, "census_tile": 256,
                        "agg_wide": wide})
         got = m.compute(left, right)
@@ -93,6 +98,9 @@ def test_unsupported_schedule_is_refused(gpu_available):
     with pytest.raises(ls.StereoException, match="census tile"):
         m.compute(left, right)
     m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 4})
+    with pytest.raises(ls.StereoException, match="64-lane scanlines need a padded range"):
+        m.compute(left, right)
+    m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 7})
     with pytest.raises(ls.StereoException, match="lane setting"):
         m.compute(left, right)
     m.setSchedule({"agg_prefetch": 4, "agg_priority": 1, "census_tile": 256, "agg_wide": 0,

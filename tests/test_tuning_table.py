@@ -61,7 +61,8 @@ def test_override_beats_table(tmp_path):
     ("640 128 1 8 16 2 256 1 2 5", "expected 7 to 9 integers"),
     ("640 128 1 8 16 2 256 x", "expected 7 to 9 integers"),
     ("640 128 1 8 16 2 256 1 x", "expected 7 to 9 integers"),
-    ("640 128 1 8 16 2 256 4", "lane setting"),
+    ("640 128 1 8 16 2 256 7", "lane setting"),
+    ("640 192 1 8 8 2 256 4", "multiple of 128"),
     ("640 128 1 8 16 2 256 1 1", "occupancy"),
     ("640 128 1 8 16 2 256 1 5", "occupancy"),
     ("640 100 1 8 8 1 256", "bad width or padded range"),
