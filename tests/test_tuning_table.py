@@ -105,7 +105,8 @@ def test_missing_file_is_an_error():
 
 def test_lane_setting_column(tmp_path):
     """The optional eighth column is the aggregation lane setting (0 when absent); every padded
-    range takes every setting with every prefetch depth its 16-lane scanlines take."""
+    range takes every 32-lane setting with every prefetch depth its 16-lane scanlines take, and
+    the 64-lane ones where it is a multiple of 128."""
     f = tmp_path / "t.tsv"
     f.write_text("640 128 1 8 16 2 128 3\n640 64 1 8 16 2 128\n640 192 1 8 8 1 256 2\n")
     assert ls.tuning_load(str(f)) == 3
@@ -122,6 +123,15 @@ def test_lane_setting_column(tmp_path):
             except ls.StereoException:
                 ok = False
             assert ok == tune.valid_prefetch(Dp // 16, pf), (Dp, pf)
+            # 64-lane settings only where a lane holds whole packed pairs (Dp = 128 k).
+            g.write_text(f"640 {Dp} 1 4 {pf} 1 256 {4 + Dp % 3}\n")
+            try:
+                ls.tuning_load(str(g))
+                ok = True
+            except ls.StereoException:
+                ok = False
+            assert ok == (Dp % 128 == 0 and tune.valid_prefetch(Dp // 16, pf)), (Dp, pf)
+            assert (4 + Dp % 3 in tune.wide_settings(Dp // 16)) == (Dp % 128 == 0), Dp
 
 
 def test_occupancy_column(tmp_path):
